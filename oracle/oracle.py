@@ -1,0 +1,165 @@
+"""ctypes front end of the C restatement (``oracle/bfvmb_oracle.c``) -- test infrastructure.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg use this module,
+and only as the checker; the product path (biofvm-b_amd/) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbfvmb_oracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_bp = ctypes.POINTER(ctypes.c_ubyte)
+
+
+def build() -> None:
+    subprocess.run(["make", "-C", HERE, "-s", "libbfvmb_oracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        i, d = ctypes.c_int, ctypes.c_double
+        L.orc_lod_constants.argtypes = [i, _dp, _dp, d, d, _dp, _dp, _dp, _dp]
+        L.orc_thomas_axis.argtypes = [i, i, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_apply_dirichlet.argtypes = [i, i, i, i, _dp, _dp, _bp]
+        L.orc_sweep.argtypes = [i, i, i, i, i, _dp, _dp, _dp, _dp]
+        L.orc_lod_step.argtypes = [i, i, i, i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _bp]
+        L.orc_agent_constants.argtypes = [i, d, d, d, _dp, _dp, _dp, _dp, _dp]
+        L.orc_sources_sinks.argtypes = [i, i, _ip, _bp, _dp, _dp, _dp]
+        L.orc_voxel_index.argtypes = [_dp, _dp, d, d, d, i, i, i]
+        L.orc_voxel_index.restype = i
+        L.orc_run.argtypes = [i, i, i, i, d, d, _dp, _dp, _dp, _bp, i, _ip, _bp, _dp, _dp, i, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def _b(a):
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    return a, a.ctypes.data_as(_bp)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+def lod_constants(D, decay, dt, dx):
+    D, pD = _d(D)
+    dc, pdc = _d(decay)
+    n = D.size
+    out = [np.empty(n) for _ in range(4)]
+    lib().orc_lod_constants(n, pD, pdc, dt, dx, *[o.ctypes.data_as(_dp) for o in out])
+    return out  # c1, c2, c3, c3a
+
+
+def thomas_axis(n, c1, c2, c3, c3a):
+    nsub = len(c1)
+    denom = np.empty((n, nsub))
+    c = np.empty((n, nsub))
+    args = [_d(a)[1] for a in (c1, c2, c3, c3a)]
+    keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (c1, c2, c3, c3a)]
+    args = [k.ctypes.data_as(_dp) for k in keep]
+    lib().orc_thomas_axis(n, nsub, *args, denom.ctypes.data_as(_dp), c.ctypes.data_as(_dp))
+    return denom, c
+
+
+def apply_dirichlet(rho, dvalue, dactive):
+    nx, ny, nz, nsub = rho.shape
+    dv, pdv = _d(dvalue)
+    da, pda = _b(dactive)
+    lib().orc_apply_dirichlet(nx, ny, nz, nsub, rho.ctypes.data_as(_dp), pdv, pda)
+
+
+def sweep(axis, rho, c1, denom, c):
+    nx, ny, nz, nsub = rho.shape
+    keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (c1, denom, c)]
+    lib().orc_sweep(axis, nx, ny, nz, nsub, rho.ctypes.data_as(_dp), *[k.ctypes.data_as(_dp) for k in keep])
+
+
+def agent_constants(dt, volume, voxel_volume, S, U, T):
+    S, U, T = (np.ascontiguousarray(a, dtype=np.float64) for a in (S, U, T))
+    n, nsub = S.shape
+    t1 = np.empty((n, nsub))
+    t2 = np.empty((n, nsub))
+    vol = np.broadcast_to(np.asarray(volume, dtype=np.float64), (n,))
+    for a in range(n):
+        lib().orc_agent_constants(nsub, dt, float(vol[a]), voxel_volume, S[a].ctypes.data_as(_dp),
+                                  U[a].ctypes.data_as(_dp), T[a].ctypes.data_as(_dp),
+                                  t1[a].ctypes.data_as(_dp), t2[a].ctypes.data_as(_dp))
+    return t1, t2
+
+
+def voxel_indices(case):
+    """Reference agent -> voxel mapping; out-of-box agents stay at voxel 0 (active)."""
+    bbox, pbbox = _d(case.bbox)
+    out = np.empty(case.nagents, dtype=np.int32)
+    for a in range(case.nagents):
+        p, pp = _d(case.pos[a])
+        v = lib().orc_voxel_index(pp, pbbox, case.dx, case.dx, case.dx, case.nx, case.ny, case.nz)
+        out[a] = 0 if v < 0 else v
+    return out
+
+
+def agent_tables(case):
+    voxel = voxel_indices(case)
+    dV = (case.dx * case.dx) * case.dx
+    t1, t2 = agent_constants(case.dt, case.volume, dV, case.S, case.U, case.T)
+    return voxel, t1, t2
+
+
+def sources_sinks(rho, voxel, active, t1, t2):
+    nsub = rho.shape[-1]
+    v, pv = _i(voxel)
+    a, pa = _b(active)
+    k1, p1 = _d(t1)
+    k2, p2 = _d(t2)
+    lib().orc_sources_sinks(len(v), nsub, pv, pa, p1, p2, rho.ctypes.data_as(_dp))
+
+
+def run(case, nsteps=None, rho=None, with_agents=None):
+    """Run ``case`` with the C restatement; returns the AoS field after ``nsteps``."""
+    nsteps = case.nsteps if nsteps is None else nsteps
+    with_agents = case.with_agents if with_agents is None else with_agents
+    rho = np.array(case.init if rho is None else rho, dtype=np.float64, order="C", copy=True)
+    voxel, t1, t2 = agent_tables(case)
+    na = case.nagents if with_agents else 0
+    active = np.ones(max(na, 1), dtype=np.uint8)
+    D, pD = _d(case.D)
+    dc, pdc = _d(case.decay)
+    dv, pdv = _d(case.dvalue)
+    da, pda = _b(case.dactive)
+    v, pv = _i(voxel if na else np.zeros(1))
+    a, pa = _b(active)
+    k1, p1 = _d(t1 if na else np.zeros(1))
+    k2, p2 = _d(t2 if na else np.zeros(1))
+    work = np.empty((3 * (case.nx + case.ny + case.nz) + 4) * case.nsub)
+    lib().orc_run(case.nx, case.ny, case.nz, case.nsub, case.dx, case.dt, pD, pdc, pdv, pda, na, pv, pa,
+                  p1, p2, nsteps, rho.ctypes.data_as(_dp), work.ctypes.data_as(_dp))
+    return rho
+
+
+def run_snapshots(case):
+    """Fields at each of ``case.snaps`` (ascending), by repeated stepping."""
+    out, rho, done = {}, None, 0
+    for s in sorted(case.snaps):
+        rho = run(case, nsteps=s - done, rho=case.init if rho is None else rho)
+        done = s
+        out[s] = rho.copy()
+    return out

@@ -1,0 +1,167 @@
+// ref_driver.cpp -- runs the reference BioFVM-B library itself on one case file.
+//
+// TEST INFRASTRUCTURE ONLY: built by oracle/Makefile against the reference sources
+// (never copied here) into oracle/_ref/, it writes the golden vectors under tests/golden/
+// and serves as the "reference" CPU baseline.  Nothing in the product links it.
+//
+// The set-up follows the harness order the reference's own drivers use
+// (examples/test_VS.cpp:118-245, examples/complete_diffusion_demo.cpp:223-240,430-493):
+//   resize_densities + set_density, resize_space_uniform (MPI overload, world size 1),
+//   Dirichlet value/activation per substrate, initial field copied into p_density_vectors,
+//   agents via create_basic_agent / register_microenvironment / assign_position /
+//   set_total_volume / rates / set_internal_uptake_constants, then per step
+//   simulate_cell_sources_and_sinks(dt) followed by simulate_diffusion_decay(...).
+//
+// usage: ref_driver <case.bin> <out.bin> [lod|avx] [granularity] [--time]
+#include <mpi.h>
+#include <omp.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "BioFVM.h"
+
+using namespace BioFVM;
+
+namespace {
+
+// Basic_Agent keeps the source/sink constants protected; a derived class may name them
+// through a pointer to member of the base, which is then applied to any Basic_Agent.
+struct AgentPeek : Basic_Agent {
+    static std::vector<double> Basic_Agent::*temp1() { return &AgentPeek::cell_source_sink_solver_temp1; }
+    static std::vector<double> Basic_Agent::*temp2() { return &AgentPeek::cell_source_sink_solver_temp2; }
+};
+
+struct Reader {
+    FILE *f;
+    template <class T> void get(T *p, size_t n) {
+        if (n && fread(p, sizeof(T), n, f) != n) {
+            fprintf(stderr, "ref_driver: short read\n");
+            exit(2);
+        }
+    }
+    template <class T> T one() { T v; get(&v, 1); return v; }
+};
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    if (argc < 3) {
+        fprintf(stderr, "usage: %s case.bin out.bin [lod|avx] [granularity] [--time]\n", argv[0]);
+        return 2;
+    }
+    std::string solver = argc > 3 ? argv[3] : "lod";
+    int gran = argc > 4 ? atoi(argv[4]) : 1;
+    bool timing = argc > 5 && std::string(argv[5]) == "--time";
+
+    int provided = 0;
+    MPI_Init_thread(nullptr, nullptr, MPI_THREAD_FUNNELED, &provided);
+    int size = 0, rank = 0;
+    MPI_Comm_size(MPI_COMM_WORLD, &size);
+    MPI_Comm_rank(MPI_COMM_WORLD, &rank);
+    int dims[3] = {1, size, 1}, periodic[3] = {0, 0, 0}, coords[3] = {0, 0, 0};
+    MPI_Comm cart;
+    MPI_Cart_create(MPI_COMM_WORLD, 3, dims, periodic, 0, &cart);
+    MPI_Cart_coords(cart, rank, 3, coords);
+
+    Reader r{fopen(argv[1], "rb")};
+    if (!r.f) { perror(argv[1]); return 2; }
+    char magic[8];
+    r.get(magic, 8);
+    if (memcmp(magic, "BFVMCAS1", 8) != 0) { fprintf(stderr, "bad case magic\n"); return 2; }
+    int nx = r.one<int>(), ny = r.one<int>(), nz = r.one<int>(), nsub = r.one<int>();
+    int nsteps = r.one<int>(), with_agents = r.one<int>(), nagents = r.one<int>(), nsnap = r.one<int>();
+    double x0 = r.one<double>(), y0 = r.one<double>(), z0 = r.one<double>();
+    double dx = r.one<double>(), dt = r.one<double>();
+    std::vector<double> D(nsub), decay(nsub), dval(nsub);
+    std::vector<int> dact(nsub);
+    r.get(D.data(), nsub); r.get(decay.data(), nsub); r.get(dval.data(), nsub); r.get(dact.data(), nsub);
+    size_t nvox = (size_t)nx * ny * nz;
+    std::vector<double> init(nvox * nsub);
+    r.get(init.data(), init.size());
+    std::vector<double> pos(3 * (size_t)nagents), vol(nagents), S((size_t)nagents * nsub),
+        U((size_t)nagents * nsub), T((size_t)nagents * nsub);
+    r.get(pos.data(), pos.size()); r.get(vol.data(), vol.size());
+    r.get(S.data(), S.size()); r.get(U.data(), U.size()); r.get(T.data(), T.size());
+    std::vector<int> snaps(nsnap);
+    r.get(snaps.data(), nsnap);
+    fclose(r.f);
+
+    Microenvironment M;
+    M.resize_densities(nsub);
+    for (int d = 0; d < nsub; ++d)
+        M.set_density(d, "substrate" + std::to_string(d), "dimensionless", D[d], decay[d]);
+    M.resize_space_uniform(x0, x0 + nx * dx, y0, y0 + ny * dx, z0, z0 + nz * dx, dx, dims, coords);
+    if (M.mesh.x_size != nx || M.mesh.y_size != ny || M.mesh.z_size != nz) {
+        fprintf(stderr, "mesh size mismatch %d %d %d\n", M.mesh.x_size, M.mesh.y_size, M.mesh.z_size);
+        return 3;
+    }
+    for (int d = 0; d < nsub; ++d) {
+        M.set_substrate_dirichlet_value(d, dval[d]);
+        M.set_substrate_dirichlet_activation(d, dact[d] != 0);
+    }
+    typedef void (*slot_t)(Microenvironment &, double, int, int, int *, int *, MPI_Comm);
+    slot_t lod = diffusion_decay_solver__constant_coefficients_LOD_3D;
+    slot_t avx = diffusion_decay_solver__constant_coefficients_LOD_3D_AVX256D;
+    M.diffusion_decay_solver = solver == "avx" ? avx : lod;
+    M.granurality = gran;
+    std::copy(init.begin(), init.end(), M.p_density_vectors->begin());
+
+    std::vector<int> voxel(nagents);
+    std::vector<double> t1((size_t)nagents * nsub), t2((size_t)nagents * nsub);
+    for (int a = 0; a < nagents; ++a) {
+        Basic_Agent *ag = create_basic_agent();
+        ag->register_microenvironment(&M);
+        ag->assign_position(pos[3 * a], pos[3 * a + 1], pos[3 * a + 2], rank, dims);
+        ag->set_total_volume(vol[a]);
+        for (int d = 0; d < nsub; ++d) {
+            (*ag->secretion_rates)[d] = S[(size_t)a * nsub + d];
+            (*ag->uptake_rates)[d] = U[(size_t)a * nsub + d];
+            (*ag->saturation_densities)[d] = T[(size_t)a * nsub + d];
+        }
+        ag->set_internal_uptake_constants(dt);
+        voxel[a] = ag->get_current_voxel_index();
+        const std::vector<double> &a1 = ag->*AgentPeek::temp1();
+        const std::vector<double> &a2 = ag->*AgentPeek::temp2();
+        for (int d = 0; d < nsub; ++d) {
+            t1[(size_t)a * nsub + d] = a1[d];
+            t2[(size_t)a * nsub + d] = a2[d];
+        }
+    }
+
+    FILE *out = fopen(argv[2], "wb");
+    if (!out) { perror(argv[2]); return 2; }
+    fwrite("BFVMOUT1", 1, 8, out);
+    fwrite(&nagents, sizeof(int), 1, out);
+    fwrite(voxel.data(), sizeof(int), nagents, out);
+    fwrite(t1.data(), sizeof(double), t1.size(), out);
+    fwrite(t2.data(), sizeof(double), t2.size(), out);
+    fwrite(&nsnap, sizeof(int), 1, out);
+
+    int next = 0;
+    double total_ms = 0.0;
+    for (int s = 1; s <= nsteps; ++s) {
+        auto t_a = std::chrono::steady_clock::now();
+        if (with_agents)
+            M.simulate_cell_sources_and_sinks(dt);
+        M.simulate_diffusion_decay(dt, size, rank, coords, dims, cart);
+        auto t_b = std::chrono::steady_clock::now();
+        total_ms += std::chrono::duration<double, std::milli>(t_b - t_a).count();
+        while (next < nsnap && snaps[next] == s) {
+            fwrite(&s, sizeof(int), 1, out);
+            fwrite(M.p_density_vectors->data(), sizeof(double), nvox * nsub, out);
+            ++next;
+        }
+    }
+    fclose(out);
+    if (timing)
+        printf("{\"steps\": %d, \"ms_total\": %.6f, \"ms_per_step\": %.6f, \"threads\": %d}\n", nsteps,
+               total_ms, total_ms / nsteps, omp_get_max_threads());
+    MPI_Finalize();
+    return 0;
+}
