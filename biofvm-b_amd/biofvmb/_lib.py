@@ -1,0 +1,81 @@
+"""ctypes binding of the C ABI in include/biofvmb.h (libbiofvmb.so, built in-tree).
+
+The library is the only compute path: when it is missing or no gfx950 device is present
+the calls fail with an error, they never fall back to a CPU implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libbiofvmb.so")
+HEADER_PATH = os.path.join(os.path.dirname(PKG_DIR), "include", "biofvmb.h")
+
+OK, ERR_ARG, ERR_STATE, ERR_HIP, ERR_NOMEM, ERR_NODEV = 0, 1, 2, 3, 4, 5
+FP64, FP32 = 0, 1
+
+_c = ctypes
+_dp = _c.POINTER(_c.c_double)
+_ip = _c.POINTER(_c.c_int)
+_bp = _c.POINTER(_c.c_uint8)
+_vp = _c.c_void_p
+
+# name -> (restype, argtypes); mirrors include/biofvmb.h one to one
+SIGNATURES = {
+    "bfvmb_version": (_c.c_char_p, []),
+    "bfvmb_create": (_c.c_int, [_c.POINTER(_vp), _c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_double, _c.c_int, _c.c_int]),
+    "bfvmb_destroy": (_c.c_int, [_vp]),
+    "bfvmb_last_error": (_c.c_char_p, [_vp]),
+    "bfvmb_set_stream": (_c.c_int, [_vp, _vp]),
+    "bfvmb_set_coefficients": (_c.c_int, [_vp, _dp, _dp, _c.c_double]),
+    "bfvmb_set_dirichlet": (_c.c_int, [_vp, _dp, _bp]),
+    "bfvmb_upload_density": (_c.c_int, [_vp, _dp]),
+    "bfvmb_download_density": (_c.c_int, [_vp, _dp]),
+    "bfvmb_set_agents": (_c.c_int, [_vp, _c.c_int, _ip, _dp, _dp, _bp]),
+    "bfvmb_set_agent_rates": (_c.c_int, [_vp, _c.c_int, _ip, _dp, _dp, _dp, _dp, _bp, _c.c_double]),
+    "bfvmb_voxel_indices": (_c.c_int, [_vp, _c.c_int, _dp, _dp, _ip]),
+    "bfvmb_simulate_cell_sources_and_sinks": (_c.c_int, [_vp]),
+    "bfvmb_simulate_diffusion_decay": (_c.c_int, [_vp, _c.c_int]),
+    "bfvmb_run": (_c.c_int, [_vp, _c.c_int]),
+    "bfvmb_sync": (_c.c_int, [_vp]),
+    "bfvmb_launch_phase": (_c.c_int, [_vp, _c.c_int, _c.c_int]),
+    "bfvmb_set_launch": (_c.c_int, [_vp, _c.c_int, _c.c_int]),
+    "bfvmb_density_device_ptr": (_c.c_int, [_vp, _c.POINTER(_vp)]),
+}
+
+_lib = None
+
+
+class BioFVMBError(RuntimeError):
+    pass
+
+
+def load(path: str = LIB_PATH):
+    """Load libbiofvmb.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise BioFVMBError(f"{path} is missing: build it with `make -C biofvm-b_amd` "
+                               "or __graft_entry__.build()")
+        lib = _c.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def dptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a: np.ndarray):
+    return a.ctypes.data_as(_ip)
+
+
+def bptr(a: np.ndarray):
+    return a.ctypes.data_as(_bp)

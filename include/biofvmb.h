@@ -1,0 +1,118 @@
+/*
+ * biofvmb.h -- C ABI of the MI355X (gfx950) BioFVM-B hot path: the constant-coefficient
+ * LOD diffusion-decay step and the agent source/sink update.
+ *
+ * One context holds one microenvironment resident on one GPU, as [substrate][z][y][x]
+ * arrays (x contiguous) in FP64 or FP32.  Host arrays cross the ABI in the reference's
+ * own voxel-major layout, rho[((x*ny + y)*nz + z)*nsub + d]  (BioFVM_mesh.cpp:659-665,
+ * BioFVM_parallel.cpp:34-39), and are transposed on the device.
+ *
+ * Conventions
+ *   - every entry point returns an int status (BFVMB_OK == 0); no C++ exception crosses
+ *     the ABI; bfvmb_last_error() gives the message of the last failure of a context;
+ *   - host pointers are borrowed for the duration of the call only;
+ *   - device memory is owned by the context; work is ordered on the context's stream
+ *     (its own, or the one given to bfvmb_set_stream); one host thread per context;
+ *   - the host copy of the field is stale after a step until bfvmb_download_density.
+ */
+#ifndef BIOFVMB_H
+#define BIOFVMB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BFVMB_OK 0
+#define BFVMB_ERR_ARG 1     /* bad argument or size */
+#define BFVMB_ERR_STATE 2   /* call out of order (e.g. step before coefficients) */
+#define BFVMB_ERR_HIP 3     /* HIP runtime failure */
+#define BFVMB_ERR_NOMEM 4   /* device allocation failed */
+#define BFVMB_ERR_NODEV 5   /* no usable gfx950 device */
+
+#define BFVMB_FP64 0        /* drop-in path: FP64 storage and sweeps */
+#define BFVMB_FP32 1        /* bulk path: FP32 storage and sweeps, FP64 Thomas set-up */
+
+typedef struct bfvmb_ctx bfvmb_ctx;
+
+/* Library version string. */
+const char *bfvmb_version(void);
+
+/* Create a context for an nx*ny*nz grid with nsub substrates and voxel edge dx on HIP
+ * device `device`.  Replaces the sizing done by Microenvironment::resize_densities +
+ * resize_space_uniform (BioFVM_microenvironment.cpp:709-749, BioFVM_parallel.cpp:18-59):
+ * the field starts at 0; Dirichlet starts at value 100.0, all substrates active. */
+int bfvmb_create(bfvmb_ctx **out, int nx, int ny, int nz, int nsub, double dx, int precision,
+                 int device);
+int bfvmb_destroy(bfvmb_ctx *ctx);
+const char *bfvmb_last_error(const bfvmb_ctx *ctx);
+
+/* Use `stream` (a hipStream_t; NULL = the context's own stream) for all later work. */
+int bfvmb_set_stream(bfvmb_ctx *ctx, void *stream);
+
+/* Thomas set-up of diffusion_decay_solver__constant_coefficients_LOD_3D's first call
+ * (BioFVM_parallel.cpp:714-918), computed on the host in FP64 and uploaded.  As in the
+ * reference, dt, D and decay are frozen here; call again to change them. */
+int bfvmb_set_coefficients(bfvmb_ctx *ctx, const double *diffusion, const double *decay,
+                           double dt);
+
+/* Dirichlet value and activation per substrate: set_substrate_dirichlet_value /
+ * set_substrate_dirichlet_activation (BioFVM_microenvironment.cpp:328-359).  Applied to all
+ * six faces before and between the sweeps, as apply_dirichlet_conditions (:381-484). */
+int bfvmb_set_dirichlet(bfvmb_ctx *ctx, const double *value, const uint8_t *active);
+
+/* Host <-> device field transfer in the reference layout (p_density_vectors). */
+int bfvmb_upload_density(bfvmb_ctx *ctx, const double *rho_voxel_major);
+int bfvmb_download_density(bfvmb_ctx *ctx, double *rho_voxel_major);
+
+/* Agents for the source/sink update: voxel index (reference numbering), per-agent
+ * cell_source_sink_solver_temp1/temp2 ([n][nsub], BioFVM_basic_agent.cpp:155-176) and the
+ * active flag.  Agents are stably sorted by voxel inside, so that agents sharing a voxel are
+ * applied in list order, as simulate_cell_sources_and_sinks does on one thread. */
+int bfvmb_set_agents(bfvmb_ctx *ctx, int n, const int *voxel, const double *temp1,
+                     const double *temp2, const uint8_t *active);
+
+/* Same, from rates: temp1/temp2 are formed as set_internal_uptake_constants does, with
+ * k = (dt*volume[a])/dV:  temp1 = (S*T)*k,  temp2 = fma(k,U,fma(k,S,1)). */
+int bfvmb_set_agent_rates(bfvmb_ctx *ctx, int n, const int *voxel, const double *volume,
+                          const double *secretion, const double *uptake,
+                          const double *saturation, const uint8_t *active, double dt);
+
+/* Host helper: voxel of each agent position ([n][3], micrometres) in a grid whose lower
+ * corner is bbox_min, as Basic_Agent::assign_position + update_voxel_index compute it
+ * (BioFVM_basic_agent.cpp:112-153, BioFVM_parallel.cpp:418-522 at world size 1):
+ * floor((p - min)/dx) clamped into the grid.  A position outside the bounding box
+ * (BioFVM_mesh.cpp:204-213) gets -1; the reference then leaves that agent at voxel 0. */
+int bfvmb_voxel_indices(const bfvmb_ctx *ctx, int n, const double *pos, const double *bbox_min,
+                        int *voxel);
+
+/* Microenvironment::simulate_cell_sources_and_sinks (BioFVM_microenvironment.cpp:1155-1169). */
+int bfvmb_simulate_cell_sources_and_sinks(bfvmb_ctx *ctx);
+
+/* nsteps calls of the diffusion_decay_solver slot (BioFVM_parallel.cpp:693-1395): per step
+ * Dirichlet, x sweep, Dirichlet, y sweep, Dirichlet, z sweep, Dirichlet. */
+int bfvmb_simulate_diffusion_decay(bfvmb_ctx *ctx, int nsteps);
+
+/* nsteps of the demo time loop (complete_diffusion_demo.cpp:489-493): per step, cell
+ * sources/sinks then one LOD step. */
+int bfvmb_run(bfvmb_ctx *ctx, int nsteps);
+
+/* Block the host until the context's stream is idle. */
+int bfvmb_sync(bfvmb_ctx *ctx);
+
+/* One kernel of the step, for measurement: phase 0 = sources/sinks, 1 = x, 2 = y, 3 = z
+ * sweep; enqueued `count` times on the context's stream. */
+int bfvmb_launch_phase(bfvmb_ctx *ctx, int phase, int count);
+
+/* Launch shape of the sweep kernels (0 = library default): resident blocks for the y/z
+ * sweeps (persistent grid) and for the x sweep.  Used by the autotuner. */
+int bfvmb_set_launch(bfvmb_ctx *ctx, int yz_blocks, int x_blocks);
+
+/* Device address of the current field, [nsub][nz][ny][nx] of 8- or 4-byte reals. */
+int bfvmb_density_device_ptr(bfvmb_ctx *ctx, void **ptr);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BIOFVMB_H */

@@ -68,6 +68,15 @@ class Case:
             np.asarray(self.snaps, dtype=np.int32).tofile(f)
 
 
+def effective_dirichlet(case: Case):
+    """(value, active) per substrate as the reference holds them: a negative activation in a
+    case means "setters not called", leaving resize_space's defaults (100.0, active)."""
+    dact = np.asarray(case.dactive)
+    value = np.where(dact < 0, 100.0, np.asarray(case.dvalue, dtype=np.float64))
+    active = np.where(dact < 0, 1, dact != 0).astype(np.uint8)
+    return value, active
+
+
 def read_output(path: str, case: Case) -> dict:
     n, nsub = case.nagents, case.nsub
     nv = case.nx * case.ny * case.nz

@@ -143,8 +143,10 @@ def run(case, nsteps=None, rho=None, with_agents=None):
     active = np.ones(max(na, 1), dtype=np.uint8)
     D, pD = _d(case.D)
     dc, pdc = _d(case.decay)
-    dv, pdv = _d(case.dvalue)
-    da, pda = _b(case.dactive)
+    from casefile import effective_dirichlet
+    dvalue, dactive = effective_dirichlet(case)
+    dv, pdv = _d(dvalue)
+    da, pda = _b(dactive)
     v, pv = _i(voxel if na else np.zeros(1))
     a, pa = _b(active)
     k1, p1 = _d(t1 if na else np.zeros(1))

@@ -102,6 +102,8 @@ int main(int argc, char **argv)
         return 3;
     }
     for (int d = 0; d < nsub; ++d) {
+        if (dact[d] < 0)
+            continue;  // keep the defaults resize_space leaves (value 100.0, active)
         M.set_substrate_dirichlet_value(d, dval[d]);
         M.set_substrate_dirichlet_activation(d, dact[d] != 0);
     }

@@ -1,0 +1,76 @@
+"""Writes the golden vectors in tests/golden/ by running the reference itself.
+
+Requires oracle/_ref/ref_driver (``make -C oracle ref``, which compiles the reference
+sources where they lie under /root/reference).  Each fixture is one .npz holding the
+case inputs and the reference's outputs: agent voxel indices, the per-agent source/sink
+constants temp1/temp2, and full density fields at the snapshot steps.  The reference's
+AVX2 solver (granularity 3) is run on the same case and must agree bit for bit with its
+scalar solver; that is recorded in the fixture.
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(HERE)), "oracle"))
+import casefile as cf  # noqa: E402
+
+
+def cases():
+    c1 = cf.make_case(50, 50, 50, 1, seed=10, nagents=1000, nsteps=100, snaps=[1, 100], D=[1e5], decay=[0.1],
+                      dvalue=[38.0], dactive=[1], init=38.0)
+    c1.U[:] = 10.0  # config 1: uptake 10/min, secretion 0
+    c1.S[:] = 0.0
+    yield "config1_50cube", c1
+    yield "aniso_17x23x31x3", cf.make_case(17, 23, 31, 3, seed=1, nagents=200, nsteps=5, snaps=[1, 5],
+                                           shared_voxels=True, outside=3)
+    yield "nx1_1x5x7x2", cf.make_case(1, 5, 7, 2, seed=2, nagents=10, nsteps=3)
+    yield "ny1_2x1x9x4", cf.make_case(2, 1, 9, 4, seed=3, nagents=30, nsteps=4, shared_voxels=True)
+    yield "nz1_6x4x1x2", cf.make_case(6, 4, 1, 2, seed=4, nagents=5, nsteps=3)
+    yield "one_voxel_1x1x1x3", cf.make_case(1, 1, 1, 3, seed=5, nagents=4, nsteps=2)
+    yield "conserve_20x18x16x2", cf.make_case(20, 18, 16, 2, seed=12, nagents=0, nsteps=10, snaps=[10],
+                                              decay=[0.0, 0.0], dactive=[0, 0], with_agents=False)
+    yield "multi_24x20x28x5", cf.make_case(24, 20, 28, 5, seed=13, nagents=1500, nsteps=12, snaps=[12],
+                                           shared_voxels=True, outside=5)
+    yield "defaults_12x10x14x2", cf.make_case(12, 10, 14, 2, seed=14, nagents=50, nsteps=4, dactive=[-1, -1])
+    yield "long_x_300x17x19x1", cf.make_case(300, 17, 19, 1, seed=7, nagents=100, nsteps=4)
+
+
+def main():
+    for name, case in cases():
+        with tempfile.TemporaryDirectory() as d:
+            out = cf.run_reference(case, d, solver="lod", granularity=1)
+            avx = cf.run_reference(case, d, solver="avx", granularity=3)
+        avx_equal = all(np.array_equal(out["fields"][s], avx["fields"][s]) for s in case.snaps)
+        arrays = dict(nx=case.nx, ny=case.ny, nz=case.nz, nsub=case.nsub, dx=case.dx, dt=case.dt, D=case.D,
+                      decay=case.decay, dvalue=case.dvalue, dactive=case.dactive, init=case.init, pos=case.pos,
+                      volume=case.volume, S=case.S, U=case.U, T=case.T, nsteps=case.nsteps,
+                      snaps=np.asarray(case.snaps), with_agents=int(case.with_agents), voxel=out["voxel"],
+                      temp1=out["temp1"], temp2=out["temp2"], avx_equal=int(avx_equal))
+        for s, f in out["fields"].items():
+            arrays[f"field_{s}"] = f
+        path = os.path.join(HERE, f"{name}.npz")
+        np.savez_compressed(path, **arrays)
+        print(f"{name}: {os.path.getsize(path) / 1e3:.0f} kB, avx==lod {avx_equal}")
+
+
+def load(name):
+    """Fixture -> (Case, reference outputs).  Loads with allow_pickle=False."""
+    z = np.load(os.path.join(HERE, f"{name}.npz"), allow_pickle=False)
+    case = cf.Case(int(z["nx"]), int(z["ny"]), int(z["nz"]), int(z["nsub"]), float(z["dx"]), float(z["dt"]),
+                   z["D"], z["decay"], z["dvalue"], z["dactive"], z["init"], z["pos"], z["volume"], z["S"], z["U"],
+                   z["T"], int(z["nsteps"]), [int(s) for s in z["snaps"]], bool(int(z["with_agents"])))
+    ref = {"voxel": z["voxel"], "temp1": z["temp1"], "temp2": z["temp2"], "avx_equal": bool(int(z["avx_equal"])),
+           "fields": {int(s): z[f"field_{int(s)}"] for s in z["snaps"]}}
+    return case, ref
+
+
+NAMES = ["config1_50cube", "aniso_17x23x31x3", "nx1_1x5x7x2", "ny1_2x1x9x4", "nz1_6x4x1x2", "one_voxel_1x1x1x3",
+         "conserve_20x18x16x2", "multi_24x20x28x5", "defaults_12x10x14x2", "long_x_300x17x19x1"]
+
+if __name__ == "__main__":
+    main()

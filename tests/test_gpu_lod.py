@@ -1,0 +1,131 @@
+"""GPU parity of the LOD step and the agent source/sink update against the oracle.
+
+The FP64 path must be bit-identical to the C restatement (itself bit-identical to the
+compiled reference, see test_oracle.py); the FP32 path is held to 1e-5 max relative
+error against the FP64 oracle, with the error printed.
+"""
+import numpy as np
+import pytest
+
+import casefile as cf
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(case, precision):
+    from biofvmb import LODSolver
+    s = LODSolver(case.nx, case.ny, case.nz, case.nsub, case.dx, precision)
+    s.set_coefficients(case.D, case.decay, case.dt)
+    s.set_dirichlet(case.dvalue, case.dactive)
+    s.upload(case.init)
+    if case.with_agents and case.nagents:
+        voxel = orc.voxel_indices(case)
+        s.set_agent_rates(voxel, case.volume, case.S, case.U, case.T, case.dt)
+    return s
+
+
+def gpu_run(case, precision="fp64", nsteps=None):
+    s = _solver(case, precision)
+    s.run(case.nsteps if nsteps is None else nsteps)
+    out = s.download()
+    s.close()
+    return out
+
+
+def rel_err(a, b):
+    den = np.maximum(np.abs(b), 1e-300)
+    r = np.abs(a - b) / den
+    return float(r.max()), float(np.sqrt(np.mean(r * r)))
+
+
+CASES = {
+    "aniso_17x23x31x3": dict(nx=17, ny=23, nz=31, nsub=3, seed=1, nagents=200, nsteps=5, shared_voxels=True, outside=3),
+    "nx1": dict(nx=1, ny=5, nz=7, nsub=2, seed=2, nagents=10, nsteps=3),
+    "ny1": dict(nx=2, ny=1, nz=9, nsub=4, seed=3, nagents=30, nsteps=4, shared_voxels=True),
+    "nz1": dict(nx=6, ny=4, nz=1, nsub=2, seed=4, nagents=5, nsteps=3),
+    "one_voxel": dict(nx=1, ny=1, nz=1, nsub=3, seed=5, nagents=4, nsteps=2),
+    "chunks_130x70x33x2": dict(nx=130, ny=70, nz=33, nsub=2, seed=6, nagents=3000, nsteps=6, shared_voxels=True),
+    "long_x_300x17x19x1": dict(nx=300, ny=17, nz=19, nsub=1, seed=7, nagents=100, nsteps=4),
+    "all_dirichlet_64x48x40x4": dict(nx=64, ny=48, nz=40, nsub=4, seed=8, nagents=2000, nsteps=8, dactive=[1, 1, 1, 1]),
+    "no_dirichlet_40x41x42x2": dict(nx=40, ny=41, nz=42, nsub=2, seed=9, nagents=500, nsteps=8, dactive=[0, 0]),
+    "config1_50cube_100steps": dict(nx=50, ny=50, nz=50, nsub=1, seed=10, nagents=1000, nsteps=100, D=[1e5], decay=[0.1],
+                                    dvalue=[38.0], dactive=[1], init=38.0),
+}
+
+
+def _case(name):
+    kw = dict(CASES[name])
+    c = cf.make_case(**kw)
+    if name == "config1_50cube_100steps":  # uptake 10/min, secretion 0
+        c.U[:] = 10.0
+        c.S[:] = 0.0
+    return c
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fp64_bitwise_vs_oracle(name):
+    case = _case(name)
+    print(f"[gpu] fp64 {name} shape={case.nx, case.ny, case.nz, case.nsub} steps={case.nsteps}", flush=True)
+    ref = orc.run(case)
+    got = gpu_run(case, "fp64")
+    assert got.shape == ref.shape
+    if not np.array_equal(got, ref):
+        mx, rms = rel_err(got, ref)
+        bad = np.argwhere(got != ref)
+        pytest.fail(f"{name}: {len(bad)} values differ, first {bad[:5].tolist()}, max rel {mx:.3e}, rms {rms:.3e}")
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fp32_error_vs_fp64_oracle(name):
+    case = _case(name)
+    print(f"[gpu] fp32 {name}", flush=True)
+    ref = orc.run(case)
+    got = gpu_run(case, "fp32")
+    mx, rms = rel_err(got, ref)
+    print(f"fp32 {name}: max rel {mx:.3e} rms rel {rms:.3e}")
+    assert mx <= 1e-5, (mx, rms)
+
+
+def test_phases_separately_match_oracle():
+    """Sources/sinks alone, then diffusion alone, as two separate entry points."""
+    case = _case("aniso_17x23x31x3")
+    s = _solver(case, "fp64")
+    voxel, t1, t2 = orc.agent_tables(case)
+    rho = case.init.copy()
+    orc.sources_sinks(rho, voxel, np.ones(case.nagents, np.uint8), t1, t2)
+    s.simulate_cell_sources_and_sinks()
+    assert np.array_equal(s.download(), rho)
+    ref = orc.run(case, nsteps=3, rho=rho, with_agents=False)
+    s.simulate_diffusion_decay(3)
+    assert np.array_equal(s.download(), ref)
+    s.close()
+
+
+def test_set_agents_with_temps_and_inactive():
+    """bfvmb_set_agents with precomputed temps; inactive agents are skipped."""
+    case = _case("chunks_130x70x33x2")
+    voxel, t1, t2 = orc.agent_tables(case)
+    active = (np.arange(case.nagents) % 3 != 0).astype(np.uint8)
+    rho = case.init.copy()
+    orc.sources_sinks(rho, voxel, active, t1, t2)
+    from biofvmb import LODSolver
+    s = LODSolver(case.nx, case.ny, case.nz, case.nsub, case.dx, "fp64")
+    s.set_coefficients(case.D, case.decay, case.dt)
+    s.upload(case.init)
+    s.set_agents(voxel, t1, t2, active)
+    s.simulate_cell_sources_and_sinks()
+    assert np.array_equal(s.download(), rho)
+    s.close()
+
+
+def test_launch_shapes_do_not_change_results():
+    """Bounded persistent grids give the same bits as one block per tile."""
+    case = _case("chunks_130x70x33x2")
+    ref = gpu_run(case, "fp64")
+    s = _solver(case, "fp64")
+    s.set_launch(7, 5)
+    s.run(case.nsteps)
+    got = s.download()
+    s.close()
+    assert np.array_equal(got, ref)
