@@ -42,7 +42,8 @@ SIGNATURES = {
     "bfvmb_run": (_c.c_int, [_vp, _c.c_int]),
     "bfvmb_sync": (_c.c_int, [_vp]),
     "bfvmb_launch_phase": (_c.c_int, [_vp, _c.c_int, _c.c_int]),
-    "bfvmb_set_launch": (_c.c_int, [_vp, _c.c_int, _c.c_int]),
+    "bfvmb_set_launch": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_int]),
+    "bfvmb_device_info": (_c.c_int, [_c.c_int, _ip, _ip]),
     "bfvmb_density_device_ptr": (_c.c_int, [_vp, _c.POINTER(_vp)]),
 }
 

@@ -107,8 +107,8 @@ class LODSolver:
                     "voxel_indices")
         return out
 
-    def set_launch(self, yz_blocks: int = 0, x_blocks: int = 0) -> None:
-        self._check(self._lib.bfvmb_set_launch(self._h, yz_blocks, x_blocks), "set_launch")
+    def set_launch(self, yz_blocks: int = 0, x_blocks: int = 0, lds_blocks_per_cu: int = 0) -> None:
+        self._check(self._lib.bfvmb_set_launch(self._h, yz_blocks, x_blocks, lds_blocks_per_cu), "set_launch")
 
     # -- stepping -------------------------------------------------------------------------
     def simulate_cell_sources_and_sinks(self) -> None:

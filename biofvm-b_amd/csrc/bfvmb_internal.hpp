@@ -12,21 +12,20 @@
 
 namespace bfvmb {
 
-// Per-axis Thomas tables in the storage type, [nsub][n] (substrate-major so that one
-// substrate's table is contiguous and uniform across a wavefront).
-template <class T>
+// Per-axis Thomas tables, FP64 on both storage paths, [nsub][n] (substrate-major so that
+// one substrate's table is contiguous and uniform across a wavefront).
 struct AxisTables {
-    const T *den;   // denominators  (thomas_denom{x,y,z})
-    const T *cc;    // upper factors (thomas_c{x,y,z})
-    const T *rden;  // 1/den, used by the FP32 path only
+    const double *den;   // denominators  (thomas_denom{x,y,z})
+    const double *cc;    // upper factors (thomas_c{x,y,z})
+    const double *rden;  // RN(1/den)
 };
 
 template <class T>
 struct SweepArgs {
     const T *__restrict__ in;   // field before the sweep   [nsub][nz][ny][nx]
     T *__restrict__ out;        // field after the sweep (a different buffer)
-    const T *c1;                // [nsub]  D*dt/dx^2
-    AxisTables<T> tab;          // tables of the swept axis
+    const double *c1;           // [nsub]  D*dt/dx^2
+    AxisTables tab;             // tables of the swept axis
     const T *dval;              // [nsub]  Dirichlet value
     const uint8_t *dact;        // [nsub]  Dirichlet activation
     int nx, ny, nz, nsub;
@@ -56,8 +55,8 @@ struct bfvmb_ctx {
     void *field[2] = {nullptr, nullptr};  // ping-pong field buffers
     int cur = 0;                          // which buffer holds the current field
 
-    // coefficient block (storage type): c1[nsub] | per axis den, cc, rden [nsub][n]
-    void *coef = nullptr;
+    // coefficient block (FP64): c1[nsub] | per axis den, cc, rden [nsub][n]
+    double *coef = nullptr;
     bool have_coef = false;
     double dt = 0.0;
     std::vector<double> D, decay;
@@ -71,6 +70,8 @@ struct bfvmb_ctx {
 
     void *stage = nullptr;  // staging for AoS<->SoA transfers (double)
     int yz_blocks = 0, x_blocks = 0;
+    int lds_blocks_per_cu = 2;        // sweep wavefronts sharing one CU's LDS
+    size_t max_lds_per_block = 0;     // device limit on LDS per workgroup
 
     std::string err;
 };
@@ -90,7 +91,10 @@ void lod_constants(int nsub, const double *D, const double *decay, double dt, do
 void thomas_axis(int n, int nsub, const double *c1, const double *c2, const double *c3,
                  const double *c3a, double *den, double *cc);
 
-// Offsets inside the coefficient block, in elements of the storage type.
+// Opt the sweep kernels into the device's full LDS size (lod.hip).
+int sweep_lds_attributes(size_t bytes);
+
+// Offsets inside the coefficient block, in doubles.
 struct CoefLayout {
     size_t c1, den[3], cc[3], rden[3], total;
 };

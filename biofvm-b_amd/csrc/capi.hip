@@ -163,6 +163,7 @@ int bfvmb_create(bfvmb_ctx **out, int nx, int ny, int nz, int nsub, double dx, i
     ctx->elem = precision == BFVMB_FP64 ? 8 : 4;
     ctx->dval.assign(nsub, 100.0);
     ctx->dact.assign(nsub, 1);
+    ctx->max_lds_per_block = prop.sharedMemPerBlock;
     int rc = BFVMB_OK;
     const size_t bytes = ctx->nvox * nsub * ctx->elem;
     if ((rc = hip_check(ctx, hipSetDevice(device), "hipSetDevice")) ||
@@ -176,7 +177,8 @@ int bfvmb_create(bfvmb_ctx **out, int nx, int ny, int nz, int nsub, double dx, i
         return rc;
     }
     ctx->stream = ctx->own_stream;
-    if ((rc = hip_check(ctx, hipMemsetAsync(ctx->field[0], 0, bytes, ctx->stream), "zero field")) ||
+    if ((rc = hip_check(ctx, (hipError_t)sweep_lds_attributes(ctx->max_lds_per_block), "LDS attributes")) ||
+        (rc = hip_check(ctx, hipMemsetAsync(ctx->field[0], 0, bytes, ctx->stream), "zero field")) ||
         (rc = upload_dirichlet(ctx))) {
         g_create_error = ctx->err;
         bfvmb_destroy(ctx);
@@ -224,28 +226,23 @@ int bfvmb_set_coefficients(bfvmb_ctx *ctx, const double *diffusion, const double
     std::vector<double> c1(ns), c2(ns), c3(ns), c3a(ns);
     lod_constants(ns, diffusion, decay, dt, ctx->dx, c1.data(), c2.data(), c3.data(), c3a.data());
     CoefLayout L = coef_layout(ctx);
-    std::vector<unsigned char> blob(L.total * ctx->elem);
-    auto put = [&](size_t idx, double v) {
-        if (ctx->elem == 8)
-            pack<double>(blob, idx, v);
-        else
-            pack<float>(blob, idx, v);
-    };
+    std::vector<double> blob(L.total);
     for (int d = 0; d < ns; ++d)
-        put(L.c1 + d, c1[d]);
+        blob[L.c1 + d] = c1[d];
     for (int a = 0; a < 3; ++a) {
         std::vector<double> den((size_t)ns * n[a]), cc((size_t)ns * n[a]);
         thomas_axis(n[a], ns, c1.data(), c2.data(), c3.data(), c3a.data(), den.data(), cc.data());
         for (size_t i = 0; i < den.size(); ++i) {
-            put(L.den[a] + i, den[i]);
-            put(L.cc[a] + i, cc[i]);
-            put(L.rden[a] + i, 1.0 / den[i]);
+            blob[L.den[a] + i] = den[i];
+            blob[L.cc[a] + i] = cc[i];
+            blob[L.rden[a] + i] = 1.0 / den[i];
         }
     }
+    const size_t bytes = blob.size() * sizeof(double);
     int rc;
-    if (!ctx->coef && (rc = hip_check(ctx, hipMalloc(&ctx->coef, blob.size()), "coefficient block")))
+    if (!ctx->coef && (rc = hip_check(ctx, hipMalloc((void **)&ctx->coef, bytes), "coefficient block")))
         return rc;
-    if ((rc = hip_check(ctx, hipMemcpyAsync(ctx->coef, blob.data(), blob.size(), hipMemcpyHostToDevice, ctx->stream),
+    if ((rc = hip_check(ctx, hipMemcpyAsync(ctx->coef, blob.data(), bytes, hipMemcpyHostToDevice, ctx->stream),
                         "coefficient upload")) ||
         (rc = hip_check(ctx, hipStreamSynchronize(ctx->stream), "coefficient sync")))
         return rc;
@@ -455,12 +452,25 @@ int bfvmb_launch_phase(bfvmb_ctx *ctx, int phase, int count)
     return BFVMB_OK;
 }
 
-int bfvmb_set_launch(bfvmb_ctx *ctx, int yz_blocks, int x_blocks)
+int bfvmb_set_launch(bfvmb_ctx *ctx, int yz_blocks, int x_blocks, int lds_blocks_per_cu)
 {
-    if (!ctx || yz_blocks < 0 || x_blocks < 0)
+    if (!ctx || yz_blocks < 0 || x_blocks < 0 || lds_blocks_per_cu < 0 || lds_blocks_per_cu > 8)
         return set_error(ctx, BFVMB_ERR_ARG, "bfvmb_set_launch: bad argument");
     ctx->yz_blocks = yz_blocks;
     ctx->x_blocks = x_blocks;
+    ctx->lds_blocks_per_cu = lds_blocks_per_cu > 0 ? lds_blocks_per_cu : 2;
+    return BFVMB_OK;
+}
+
+int bfvmb_device_info(int device, int *lds_per_block, int *cu_count)
+{
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+        return BFVMB_ERR_NODEV;
+    if (lds_per_block)
+        *lds_per_block = (int)prop.sharedMemPerBlock;
+    if (cu_count)
+        *cu_count = prop.multiProcessorCount;
     return BFVMB_OK;
 }
 

@@ -7,255 +7,453 @@
 // and overwrites all six domain faces with the Dirichlet value of each active substrate
 // before the x sweep and after every sweep (BioFVM_microenvironment.cpp:381-484).
 //
-// Here the field is [substrate][z][y][x] (x contiguous) and every sweep reads one buffer
-// and writes the other, so the loads of the forward pass can run ahead of its stores.
-//  * y and z sweeps: one line per lane, consecutive lanes on consecutive x (y sweep) or
-//    consecutive xy (z sweep), so each step of the recurrence is one coalesced row access;
-//    the next U rows are prefetched into registers while the current U are solved.
-//  * x sweep: lines lie along the contiguous axis; a wavefront stages 64 lines x W
-//    columns in LDS with coalesced row loads, each lane then walks its own line in LDS
-//    (transposed access, padded pitch), and the tile goes back with coalesced stores.
+// Design (the field is [substrate][z][y][x], x contiguous; every sweep reads one buffer and
+// writes the other):
+//  * One wavefront owns 64 lines.  The forward results of the tail of every line are held
+//    in a per-wavefront LDS "line store" (up to the whole 160 KB of the CU), only the head
+//    of a line longer than the store is parked in the output buffer, where it stays in the
+//    XCD's L2.  The backward pass then reads the tail from LDS.  HBM therefore sees one
+//    read and one write per voxel and sweep, the algorithmic minimum.
+//  * y and z sweeps: lane = line, consecutive lanes on consecutive x (y sweep) or xy
+//    (z sweep), so each step of the recurrence is a coalesced 512 B row access; a ring of
+//    P rows per lane is kept in flight.
+//  * x sweep: lines lie along the contiguous axis.  Chunks of 64 rows x 16 columns are
+//    loaded with coalesced row segments (several chunks in flight), transposed through an
+//    LDS staging tile, and each lane walks its own row.  Final values go back the same way.
+//  * FP64 division: q = a*r, then two corrections e = fma(-q,d,a), q = fma(e,r,q) with
+//    r = RN(1/d) from the host.  With q faithful after the first correction, Markstein's
+//    theorem makes the second result the correctly rounded a/d, i.e. bit-identical to the
+//    reference's division, with a 6-deep instead of a ~10-deep dependent chain.  Numerators
+//    below 2^-960 (outside the theorem's no-underflow range) take a true division.
+//  * FP32 storage path: the recurrence runs in FP64 on values converted on load (with the
+//    FP64 reciprocal), results are rounded to FP32 once when stored.
 //  * Dirichlet is fused: the x sweep applies it on load (the reference's first call) and
 //    every sweep applies it on store; a line lying entirely on an active face is written
 //    with the value directly (the reference solves it and then overwrites all of it).
-// FP64 uses true division so every value is bit-identical to the reference's; FP32 uses
-// the FP64-computed reciprocal of each denominator.
+#include <type_traits>
+
 #include "bfvmb_internal.hpp"
 
 namespace bfvmb {
 namespace {
 
-template <class T>
-__device__ __forceinline__ T fwd_step(T c1, T prev, T v, T den, T rden)
+constexpr int kWave = 64;
+
+// Thomas tables are read through the constant address space: the index is uniform, so
+// every read becomes a scalar load that never waits on the vector-memory counter that
+// tracks the in-flight field rows.
+typedef const __attribute__((address_space(4))) double *ctab;
+__device__ __forceinline__ ctab as_ctab(const double *p) { return (ctab)p; }
+
+// a/d correctly rounded from r = RN(1/d): q = a*r, two corrections (Markstein).  Valid for
+// |a| >= 2^-960 (no underflow in the corrections); callers flag smaller numerators.
+__device__ __forceinline__ double div_corr(double a, double d, double r)
 {
+    double q = a * r;
+    double e = fma(-q, d, a);
+    q = fma(e, r, q);
+    e = fma(-q, d, a);
+    return fma(e, r, q);
+}
+constexpr double kTiny = 0x1p-960;  // smallest numerator the corrected reciprocal handles
+
+// One forward step in double: y = fma(c1, prev, v) / den.  With prev = -0.0 the same
+// expression gives the first element's v / den[0] exactly, signed zeros included.
+// FP64 storage: corrected reciprocal, `amin` tracks the smallest |numerator| so that a
+// numerator outside its range can be detected once per line;
+// FP32 storage: the FP64 reciprocal.
+template <class T>
+__device__ __forceinline__ double fwd(double c1, double prev, double v, double den, double rden, double &amin)
+{
+    const double num = fma(c1, prev, v);
     if constexpr (sizeof(T) == 8) {
-        (void)rden;
-        return fma(c1, prev, v) / den;
+        amin = fmin(amin, fabs(num));
+        return div_corr(num, den, rden);
     } else {
         (void)den;
-        return fmaf(c1, prev, v) * rden;
+        return num * rden;
     }
 }
 
-template <class T>
-__device__ __forceinline__ T first_step(T v, T den, T rden)
+// Dispatch of the first P-aligned boundary: the forward pass parks [0, lo) in the output
+// buffer and keeps [lo, n) in LDS; lo is a multiple of P unless every element is parked.
+__host__ __device__ __forceinline__ int park_boundary(int n, int L, int P)
 {
-    if constexpr (sizeof(T) == 8) {
-        (void)rden;
-        return v / den;
-    } else {
-        (void)den;
-        return v * rden;
-    }
+    if (L >= n)
+        return 0;
+    if (L < P)
+        return n;
+    const int lo = (n - L + P - 1) / P * P;
+    return lo < n ? lo : n;
 }
 
-template <class T>
-__device__ __forceinline__ T back_step(T cc, T next, T v)
+// ------------------------------------------------------------------------------ y / z
+// Block = one wavefront = 64 lines of one substrate.  LDS: {den, rden}[n], cc[n] of the
+// block's substrate, then the line store [L][64].
+template <class T, int AXIS, int P>
+__global__ __launch_bounds__(64) void k_sweep_yz(SweepArgs<T> a, int L)
 {
-    if constexpr (sizeof(T) == 8)
-        return fma(-cc, next, v);
-    else
-        return fmaf(-cc, next, v);
-}
-
-// y (AXIS=1) or z (AXIS=2) sweep.  Block = 256 lanes = 256 lines of one substrate; the
-// grid strides over line tiles so that the number of resident tiles can be bounded.
-template <class T, int AXIS, int U>
-__global__ __launch_bounds__(256) void k_sweep_yz(SweepArgs<T> a)
-{
+    extern __shared__ unsigned char smem[];
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const size_t nxy = (size_t)nx * ny;
     const size_t nvox = nxy * nz;
-    const int n = AXIS == 1 ? ny : nz;                      // line length
+    const int n = AXIS == 1 ? ny : nz;
+    double2 *tab2 = reinterpret_cast<double2 *>(smem);
+    double *tabc = reinterpret_cast<double *>(tab2 + n);
+    T *ls = reinterpret_cast<T *>(tabc + n);
     const long long lines_per_s = AXIS == 1 ? (long long)nx * nz : (long long)nxy;
-    const long long tiles_per_s = (lines_per_s + 255) / 256;
+    const long long tiles_per_s = (lines_per_s + kWave - 1) / kWave;
     const long long ntiles = tiles_per_s * a.nsub;
     const size_t stride = AXIS == 1 ? (size_t)nx : nxy;
+    const int lane = threadIdx.x;
+    const int lo = park_boundary(n, L, P);
+    int tab_s = -1;
 
     for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int s = (int)(tile / tiles_per_s);
-        const long long line = (tile - (long long)s * tiles_per_s) * 256 + threadIdx.x;
-        if (line >= lines_per_s)
-            continue;
-        int x, o;  // o = z (y sweep) or y (z sweep)
-        size_t base;
-        if (AXIS == 1) {
-            o = (int)(line / nx);
-            x = (int)(line - (long long)o * nx);
-            base = (size_t)s * nvox + (size_t)o * nxy + x;
-        } else {
-            o = (int)(line / nx);
-            x = (int)(line - (long long)o * nx);
-            base = (size_t)s * nvox + (size_t)line;
+        if (s != tab_s) {  // stage this substrate's Thomas tables
+            __syncthreads();
+            const double *gd = a.tab.den + (size_t)s * n, *gr = a.tab.rden + (size_t)s * n;
+            const double *gc = a.tab.cc + (size_t)s * n;
+            for (int i = lane; i < n; i += kWave) {
+                tab2[i] = make_double2(gd[i], gr[i]);
+                tabc[i] = gc[i];
+            }
+            __syncthreads();
+            tab_s = s;
         }
+        long long line = (tile - (long long)s * tiles_per_s) * kWave + lane;
+        if (line >= lines_per_s)
+            line = lines_per_s - 1;  // idle lanes repeat the last line (same values, same stores)
+        const int o = (int)(line / nx);
+        const int x = (int)(line - (long long)o * nx);
+        const size_t base = (size_t)s * nvox + (AXIS == 1 ? (size_t)o * nxy + x : (size_t)line);
         const int on = AXIS == 1 ? nz : ny;
         const bool act = a.dact[s] != 0;
         const T dv = a.dval[s];
         const T *src = a.in + base;
         T *dst = a.out + base;
-        if (act && (x == 0 || x == nx - 1 || o == 0 || o == on - 1)) {
-            for (int k = 0; k < n; ++k)
-                dst[(size_t)k * stride] = dv;
-            continue;
-        }
-        const T c1 = a.c1[s];
-        const T *den = a.tab.den + (size_t)s * n;
-        const T *cc = a.tab.cc + (size_t)s * n;
-        const T *rden = a.tab.rden + (size_t)s * n;
+        // a line on an active face is computed along but stored as the Dirichlet value
+        const bool face = act && (x == 0 || x == nx - 1 || o == 0 || o == on - 1);
+        const double c1 = a.c1[s];
 
-        // forward elimination, U rows in flight ahead of the ones being solved
-        T cur[U];
+        // ---- forward elimination; a ring of P rows is in flight (reads past the end
+        // re-read row n-1 and are discarded)
+        T buf[P];
 #pragma unroll
-        for (int j = 0; j < U; ++j)
-            if (j < n)
-                cur[j] = src[(size_t)j * stride];
-        T y = first_step(cur[0], den[0], rden[0]);
-        dst[0] = y;
-        for (int k0 = 0; k0 < n; k0 += U) {
-            T nxt[U];
+        for (int j = 0; j < P; ++j)
+            buf[j] = src[(size_t)min(j, n - 1) * stride];
+        double y = -0.0;
+        double amin = 1.0;
+        int k0 = 0;
+        // [0, lo): parked in the output buffer
+        for (; k0 + P <= lo; k0 += P) {
+            double2 t[P];  // the chunk's table entries in one batch of LDS reads
 #pragma unroll
-            for (int j = 0; j < U; ++j) {
-                const int k = k0 + U + j;
-                if (k < n)
-                    nxt[j] = src[(size_t)k * stride];
-            }
+            for (int j = 0; j < P; ++j)
+                t[j] = tab2[k0 + j];
 #pragma unroll
-            for (int j = 0; j < U; ++j) {
+            for (int j = 0; j < P; ++j) {
                 const int k = k0 + j;
-                if (k >= 1 && k < n) {
-                    y = fwd_step(c1, y, cur[j], den[k], rden[k]);
-                    dst[(size_t)k * stride] = y;
+                const double v = (double)buf[j];
+                buf[j] = src[(size_t)min(k + P, n - 1) * stride];
+                y = fwd<T>(c1, y, v, t[j].x, t[j].y, amin);
+                dst[(size_t)k * stride] = face ? dv : (T)y;
+            }
+        }
+        if (k0 < lo) {  // only when every element is parked and n is not a multiple of P
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                const int k = k0 + j;
+                if (k < lo) {
+                    const double2 t = tab2[k];
+                    y = fwd<T>(c1, y, (double)buf[j], t.x, t.y, amin);
+                    dst[(size_t)k * stride] = face ? dv : (T)y;
                 }
             }
-#pragma unroll
-            for (int j = 0; j < U; ++j)
-                cur[j] = nxt[j];
+            k0 = lo;
         }
-        // back substitution; the last element is y[n-1] itself
-        if (act)
-            dst[(size_t)(n - 1) * stride] = dv;
-        T xv = y;
-        for (int k0 = n - 2; k0 >= 0; k0 -= U) {
-            T v[U];
+        // [lo, n): LDS line store
+        for (; k0 + P <= n; k0 += P) {
+            double2 t[P];
 #pragma unroll
-            for (int j = 0; j < U; ++j) {
-                const int k = k0 - j;
-                if (k >= 0)
-                    v[j] = dst[(size_t)k * stride];
+            for (int j = 0; j < P; ++j)
+                t[j] = tab2[k0 + j];
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                const int k = k0 + j;
+                const double v = (double)buf[j];
+                buf[j] = src[(size_t)min(k + P, n - 1) * stride];
+                y = fwd<T>(c1, y, v, t[j].x, t[j].y, amin);
+                ls[(size_t)(k - lo) * kWave + lane] = (T)y;
             }
+        }
+        if (k0 < n) {
 #pragma unroll
-            for (int j = 0; j < U; ++j) {
-                const int k = k0 - j;
-                if (k >= 0) {
-                    xv = back_step(cc[k], xv, v[j]);
-                    dst[(size_t)k * stride] = (act && k == 0) ? dv : xv;
+            for (int j = 0; j < P; ++j) {
+                const int k = k0 + j;
+                if (k < n) {
+                    const double2 t = tab2[k];
+                    y = fwd<T>(c1, y, (double)buf[j], t.x, t.y, amin);
+                    ls[(size_t)(k - lo) * kWave + lane] = (T)y;
+                }
+            }
+        }
+        if constexpr (sizeof(T) == 8) {
+            // a numerator below 2^-960 somewhere in the tile: redo the forward pass with
+            // true divisions (same results everywhere else)
+            if (__builtin_expect(__any(amin < kTiny), 0)) {
+                y = -0.0;
+                for (int k = 0; k < n; ++k) {
+                    y = fma(c1, y, (double)src[(size_t)k * stride]) / tab2[k].x;
+                    if (k >= lo)
+                        ls[(size_t)(k - lo) * kWave + lane] = (T)y;
+                    else
+                        dst[(size_t)k * stride] = face ? dv : (T)y;
+                }
+            }
+        }
+
+        // ---- back substitution; the last element is y[n-1] itself
+        double xv = y;
+        dst[(size_t)(n - 1) * stride] = act ? dv : (T)xv;
+        // LDS part, k = n-2 .. lo
+        for (int kt = n - 2; kt >= lo; kt -= P) {
+            if (kt - (P - 1) >= lo) {
+                double v[P], c[P];
+#pragma unroll
+                for (int j = 0; j < P; ++j) {
+                    v[j] = (double)ls[(size_t)(kt - j - lo) * kWave + lane];
+                    c[j] = tabc[kt - j];
+                }
+#pragma unroll
+                for (int j = 0; j < P; ++j) {
+                    const int k = kt - j;
+                    xv = fma(-c[j], xv, v[j]);
+                    dst[(size_t)k * stride] = (face || (act && k == 0)) ? dv : (T)xv;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < P; ++j) {
+                    const int k = kt - j;
+                    if (k >= lo) {
+                        xv = fma(-tabc[k], xv, (double)ls[(size_t)(k - lo) * kWave + lane]);
+                        dst[(size_t)k * stride] = (face || (act && k == 0)) ? dv : (T)xv;
+                    }
+                }
+            }
+        }
+        // parked head, k = min(lo-1, n-2) .. 0, read back through the ring
+        const int top = min(lo - 1, n - 2);
+        if (top >= 0) {
+#pragma unroll
+            for (int j = 0; j < P; ++j)
+                buf[j] = dst[(size_t)max(top - j, 0) * stride];
+            int kt = top;
+            for (; kt - (P - 1) >= 0; kt -= P) {
+#pragma unroll
+                for (int j = 0; j < P; ++j) {
+                    const int k = kt - j;
+                    const double v = (double)buf[j];
+                    buf[j] = dst[(size_t)max(k - P, 0) * stride];
+                    xv = fma(-tabc[k], xv, v);
+                    dst[(size_t)k * stride] = (face || (act && k == 0)) ? dv : (T)xv;
+                }
+            }
+            if (kt >= 0) {
+#pragma unroll
+                for (int j = 0; j < P; ++j) {
+                    const int k = kt - j;
+                    if (k >= 0) {
+                        xv = fma(-tabc[k], xv, (double)buf[j]);
+                        dst[(size_t)k * stride] = (face || (act && k == 0)) ? dv : (T)xv;
+                    }
                 }
             }
         }
     }
 }
 
-// x sweep.  Block = one wavefront = 64 lines (consecutive rows (y,z) of one substrate),
-// staged through LDS W columns at a time.  The forward results of all but the last chunk
-// are parked in the output buffer and read back by the backward pass.
-template <class T, int W>
-__global__ __launch_bounds__(64) void k_sweep_x(SweepArgs<T> a)
+// ------------------------------------------------------------------------------ x
+// Block = one wavefront = 64 consecutive rows (y,z) of one substrate.  LDS: staging tile
+// [64][W+1] and line store [64][L+1] (odd pitches keep the transposed accesses
+// conflict-free).  Column chunks are W = 16 wide; PC chunks of loads are kept in flight.
+template <class T, int PC>
+__global__ __launch_bounds__(64) void k_sweep_x(SweepArgs<T> a, int L)
 {
-    __shared__ T tile[64][W + 1];
-    constexpr int RB = 16;  // rows loaded per batch (loads in flight per lane)
+    constexpr int W = 16;
+    constexpr int RPI = kWave / W;   // rows per load instruction (4)
+    constexpr int NI = kWave / RPI;  // load instructions per chunk (16)
+    extern __shared__ unsigned char smem[];
+    T *stg = reinterpret_cast<T *>(smem);  // [64][W+1]
+    T *ls = stg + kWave * (W + 1);          // [64][L+1]
+    const int Lp = L + 1;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const size_t nvox = (size_t)nx * ny * nz;
     const int nrows = ny * nz;
-    const int tiles_per_s = (nrows + 63) / 64;
+    const int tiles_per_s = (nrows + kWave - 1) / kWave;
     const long long ntiles = (long long)tiles_per_s * a.nsub;
     const int lane = threadIdx.x;
-    const int last_c0 = ((nx - 1) / W) * W;
+    const int nch = (nx + W - 1) / W;
+    int cl = nch - L / W;  // chunks [cl, nch) live in the line store; L is a multiple of W
+    if (cl < 0)
+        cl = 0;
+    const int lrow = lane / W, lcol = lane % W;  // position of this lane in a load instruction
 
     for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int s = (int)(t / tiles_per_s);
-        const int r0 = (int)(t - (long long)s * tiles_per_s) * 64;
-        const int my_row = r0 + lane;
-        const bool row_ok = my_row < nrows;
+        const int r0 = (int)(t - (long long)s * tiles_per_s) * kWave;
+        const int rows_here = min(kWave, nrows - r0);
+        const int my_row = r0 + min(lane, rows_here - 1);
+        const bool row_ok = lane < rows_here;
         const int yy = my_row % ny, zz = my_row / ny;
         const bool act = a.dact[s] != 0;
         const T dv = a.dval[s];
         const bool whole = act && (yy == 0 || yy == ny - 1 || zz == 0 || zz == nz - 1);
-        const T *src = a.in + (size_t)s * nvox;
-        T *dst = a.out + (size_t)s * nvox;
-        const T c1 = a.c1[s];
-        const T *den = a.tab.den + (size_t)s * nx;
-        const T *cc = a.tab.cc + (size_t)s * nx;
-        const T *rden = a.tab.rden + (size_t)s * nx;
-        const int rows_here = min(64, nrows - r0);
+        const T *src = a.in + (size_t)s * nvox + (size_t)r0 * nx;
+        T *dst = a.out + (size_t)s * nvox + (size_t)r0 * nx;
+        const double c1 = a.c1[s];
+        const ctab den = as_ctab(a.tab.den + (size_t)s * nx);
+        const ctab cc = as_ctab(a.tab.cc + (size_t)s * nx);
+        const ctab rden = as_ctab(a.tab.rden + (size_t)s * nx);
 
-        T y = T(0);
-        for (int c0 = 0; c0 < nx; c0 += W) {
-            const int w = min(W, nx - c0);
-            for (int rb = 0; rb < rows_here; rb += RB) {
-                T r[RB];
+        // chunk c of this tile <-> registers: instruction i covers rows RPI*i .. RPI*i+3,
+        // columns c*W .. c*W+15; out-of-range lanes re-read an in-range element
+        auto load_chunk = [&](const T *from, int c, T *r) {
+            const int col = min(c * W + lcol, nx - 1);
 #pragma unroll
-                for (int q = 0; q < RB; ++q)
-                    if (rb + q < rows_here && lane < w)
-                        r[q] = src[(size_t)(r0 + rb + q) * nx + c0 + lane];
+            for (int i = 0; i < NI; ++i) {
+                const int rr = min(RPI * i + lrow, rows_here - 1);
+                r[i] = from[(size_t)rr * nx + col];
+            }
+        };
+        auto put_chunk = [&](const T *r) {
 #pragma unroll
-                for (int q = 0; q < RB; ++q)
-                    if (rb + q < rows_here && lane < w)
-                        tile[rb + q][lane] = r[q];
-            }
-            __syncthreads();
-            if (row_ok && !whole) {
-                for (int j = 0; j < w; ++j) {
-                    const int gx = c0 + j;
-                    T v = tile[lane][j];
-                    if (act && (gx == 0 || gx == nx - 1))
-                        v = dv;
-                    y = gx == 0 ? first_step(v, den[0], rden[0]) : fwd_step(c1, y, v, den[gx], rden[gx]);
-                    tile[lane][j] = y;
-                }
-            }
-            __syncthreads();
-            if (c0 != last_c0) {
-                for (int rr = 0; rr < rows_here; ++rr)
-                    if (lane < w)
-                        dst[(size_t)(r0 + rr) * nx + c0 + lane] = tile[rr][lane];
-                __syncthreads();
-            }
-        }
-        T xv = y;
-        for (int c0 = last_c0; c0 >= 0; c0 -= W) {
-            const int w = min(W, nx - c0);
-            if (c0 != last_c0) {
-                for (int rb = 0; rb < rows_here; rb += RB) {
-                    T r[RB];
+            for (int i = 0; i < NI; ++i)
+                stg[(RPI * i + lrow) * (W + 1) + lcol] = r[i];
+        };
+        auto store_chunk = [&](int c) {  // staging tile -> dst, coalesced
+            const int col = c * W + lcol;
 #pragma unroll
-                    for (int q = 0; q < RB; ++q)
-                        if (rb + q < rows_here && lane < w)
-                            r[q] = dst[(size_t)(r0 + rb + q) * nx + c0 + lane];
-#pragma unroll
-                    for (int q = 0; q < RB; ++q)
-                        if (rb + q < rows_here && lane < w)
-                            tile[rb + q][lane] = r[q];
-                }
-                __syncthreads();
+            for (int i = 0; i < NI; ++i) {
+                const int rr = RPI * i + lrow;
+                if (rr < rows_here && col < nx)
+                    dst[(size_t)rr * nx + col] = stg[rr * (W + 1) + lcol];
             }
-            if (row_ok) {
-                for (int j = w - 1; j >= 0; --j) {
-                    const int gx = c0 + j;
-                    if (whole) {
-                        tile[lane][j] = dv;
-                        continue;
+        };
+
+        // ---- forward; EXACT = true divisions (the rare redo for numerators < 2^-960)
+        T ring[PC][NI];
+        double y = -0.0;
+        auto forward = [&](auto exact_tag) {
+            constexpr bool EXACT = decltype(exact_tag)::value;
+            double amin = 1.0;
+#pragma unroll
+            for (int p = 0; p < PC; ++p)
+                load_chunk(src, min(p, nch - 1), ring[p]);
+            y = -0.0;
+            for (int c0 = 0; c0 < nch; c0 += PC) {
+#pragma unroll
+                for (int p = 0; p < PC; ++p) {
+                    const int c = c0 + p;
+                    if (c < nch) {
+                        put_chunk(ring[p]);
+                        load_chunk(src, min(c + PC, nch - 1), ring[p]);
+                        __syncthreads();
+                        const int w = min(W, nx - c * W);
+                        double dd[W], rd[W], v[W];
+#pragma unroll
+                        for (int j = 0; j < W; ++j) {
+                            const int kk = min(c * W + j, nx - 1);
+                            dd[j] = den[kk];
+                            rd[j] = rden[kk];
+                            const double u = (double)stg[lane * (W + 1) + j];
+                            v[j] = (act && (kk == 0 || kk == nx - 1)) ? (double)dv : u;
+                        }
+                        // forward results go to the line store or, for parked chunks, back
+                        // into the staging tile
+                        T *tgt = c >= cl ? ls + lane * Lp + (c - cl) * W : stg + lane * (W + 1);
+                        auto step = [&](int j) {
+                            if constexpr (EXACT)
+                                y = fma(c1, y, v[j]) / dd[j];
+                            else
+                                y = fwd<T>(c1, y, v[j], dd[j], rd[j], amin);
+                            tgt[j] = (T)y;
+                        };
+                        if (w == W) {
+#pragma unroll
+                            for (int j = 0; j < W; ++j)
+                                step(j);
+                        } else {
+#pragma unroll
+                            for (int j = 0; j < W; ++j)
+                                if (j < w)
+                                    step(j);
+                        }
+                        __syncthreads();
+                        if (c < cl) {
+                            store_chunk(c);  // park the forward results of the head
+                            __syncthreads();
+                        }
                     }
-                    if (gx != nx - 1)
-                        xv = back_step(cc[gx], xv, tile[lane][j]);
-                    tile[lane][j] = (act && (gx == 0 || gx == nx - 1)) ? dv : xv;
                 }
             }
+            return amin;
+        };
+        const double amin = forward(std::integral_constant<bool, false>());
+        if constexpr (sizeof(T) == 8) {
+            if (__builtin_expect(__any(amin < kTiny), 0))
+                forward(std::integral_constant<bool, true>());
+        } else {
+            (void)amin;
+        }
+
+        // ---- backward, last chunk first; parked chunks come back through the ring
+        double xv = y;
+#pragma unroll
+        for (int p = 0; p < PC; ++p)
+            load_chunk(dst, max(cl - 1 - p, 0), ring[p]);
+        int pslot = 0;
+        for (int c = nch - 1; c >= 0; --c) {
+            if (c < cl) {
+#pragma unroll
+                for (int p = 0; p < PC; ++p)
+                    if (p == pslot) {
+                        put_chunk(ring[p]);
+                        load_chunk(dst, max(c - PC, 0), ring[p]);
+                    }
+                pslot = pslot + 1 == PC ? 0 : pslot + 1;
+            }
             __syncthreads();
-            for (int rr = 0; rr < rows_here; ++rr)
-                if (lane < w)
-                    dst[(size_t)(r0 + rr) * nx + c0 + lane] = tile[rr][lane];
+            const int w = min(W, nx - c * W);
+            const T *from = c >= cl ? ls + lane * Lp + (c - cl) * W : stg + lane * (W + 1);
+            double cw[W], v[W];
+#pragma unroll
+            for (int j = 0; j < W; ++j) {
+                cw[j] = cc[min(c * W + j, nx - 1)];
+                v[j] = (double)from[j];
+            }
+            auto bstep = [&](int j) {
+                const int k = c * W + j;
+                if (k != nx - 1)
+                    xv = fma(-cw[j], xv, v[j]);
+                stg[lane * (W + 1) + j] = (whole || (act && (k == 0 || k == nx - 1))) ? dv : (T)xv;
+            };
+            if (w == W) {
+#pragma unroll
+                for (int j = W - 1; j >= 0; --j)
+                    bstep(j);
+            } else {
+#pragma unroll
+                for (int j = W - 1; j >= 0; --j)
+                    if (j < w)
+                        bstep(j);
+            }
+            __syncthreads();
+            store_chunk(c);
             __syncthreads();
         }
+        (void)row_ok;
     }
 }
 
@@ -263,7 +461,7 @@ template <class T>
 SweepArgs<T> make_args(bfvmb_ctx *ctx, int axis)
 {
     CoefLayout L = coef_layout(ctx);
-    const T *coef = static_cast<const T *>(ctx->coef);
+    const double *coef = ctx->coef;
     SweepArgs<T> a;
     a.in = static_cast<const T *>(ctx->field[ctx->cur]);
     a.out = static_cast<T *>(ctx->field[ctx->cur ^ 1]);
@@ -280,23 +478,45 @@ SweepArgs<T> make_args(bfvmb_ctx *ctx, int axis)
     return a;
 }
 
+constexpr size_t kLdsBytes = 160 * 1024;
+
 template <class T>
 int launch_sweep_t(bfvmb_ctx *ctx, int axis)
 {
     SweepArgs<T> a = make_args<T>(ctx, axis);
+    const int bpc = ctx->lds_blocks_per_cu > 0 ? ctx->lds_blocks_per_cu : 1;
+    const size_t budget = std::min(kLdsBytes / bpc, ctx->max_lds_per_block);
     if (axis == 0) {
-        constexpr int W = 64;
-        long long tiles = (long long)((ctx->ny * ctx->nz + 63) / 64) * ctx->nsub;
+        constexpr int W = 16;
+        const size_t stg = (size_t)kWave * (W + 1) * sizeof(T);
+        // line store elements per row, a multiple of W, with an odd pitch (L+1)
+        long long L = budget > stg ? (long long)((budget - stg) / (kWave * sizeof(T))) - 1 : 0;
+        L = L / W * W;
+        const long long need = ((ctx->nx + W - 1) / W) * W;
+        if (L > need)
+            L = need;
+        if (L < 0)
+            L = 0;
+        const size_t lds = stg + (size_t)kWave * (L + 1) * sizeof(T);
+        long long tiles = (long long)((ctx->ny * ctx->nz + kWave - 1) / kWave) * ctx->nsub;
         long long grid = ctx->x_blocks > 0 ? std::min<long long>(ctx->x_blocks, tiles) : tiles;
-        hipLaunchKernelGGL((k_sweep_x<T, W>), dim3((unsigned)grid), dim3(64), 0, ctx->stream, a);
+        hipLaunchKernelGGL((k_sweep_x<T, 3>), dim3((unsigned)grid), dim3(kWave), lds, ctx->stream, a, (int)L);
     } else {
+        const int n = axis == 1 ? ctx->ny : ctx->nz;
+        const size_t tables = (size_t)n * 3 * sizeof(double);
+        if (tables + 16 > budget)
+            return set_error(ctx, BFVMB_ERR_ARG, "sweep: axis too long for the LDS Thomas tables");
+        long long L = (long long)((budget - tables) / (kWave * sizeof(T)));
+        if (L > n)
+            L = n;
+        const size_t lds = tables + (size_t)kWave * L * sizeof(T);
         long long lines = axis == 1 ? (long long)ctx->nx * ctx->nz : (long long)ctx->nx * ctx->ny;
-        long long tiles = ((lines + 255) / 256) * ctx->nsub;
+        long long tiles = ((lines + kWave - 1) / kWave) * ctx->nsub;
         long long grid = ctx->yz_blocks > 0 ? std::min<long long>(ctx->yz_blocks, tiles) : tiles;
         if (axis == 1)
-            hipLaunchKernelGGL((k_sweep_yz<T, 1, 8>), dim3((unsigned)grid), dim3(256), 0, ctx->stream, a);
+            hipLaunchKernelGGL((k_sweep_yz<T, 1, 16>), dim3((unsigned)grid), dim3(kWave), lds, ctx->stream, a, (int)L);
         else
-            hipLaunchKernelGGL((k_sweep_yz<T, 2, 8>), dim3((unsigned)grid), dim3(256), 0, ctx->stream, a);
+            hipLaunchKernelGGL((k_sweep_yz<T, 2, 16>), dim3((unsigned)grid), dim3(kWave), lds, ctx->stream, a, (int)L);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
@@ -312,6 +532,20 @@ int launch_sweep(bfvmb_ctx *ctx, int axis)
     if (ctx->precision == BFVMB_FP64)
         return launch_sweep_t<double>(ctx, axis);
     return launch_sweep_t<float>(ctx, axis);
+}
+
+int sweep_lds_attributes(size_t lds_bytes)
+{
+    // allow the device's full LDS for every sweep instantiation
+    const int bytes = (int)lds_bytes;
+    hipError_t e = hipSuccess;
+    e = e ? e : hipFuncSetAttribute((const void *)k_sweep_yz<double, 1, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    e = e ? e : hipFuncSetAttribute((const void *)k_sweep_yz<double, 2, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    e = e ? e : hipFuncSetAttribute((const void *)k_sweep_yz<float, 1, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    e = e ? e : hipFuncSetAttribute((const void *)k_sweep_yz<float, 2, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    e = e ? e : hipFuncSetAttribute((const void *)k_sweep_x<double, 3>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    e = e ? e : hipFuncSetAttribute((const void *)k_sweep_x<float, 3>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    return e == hipSuccess ? 0 : (int)e;
 }
 
 }  // namespace bfvmb

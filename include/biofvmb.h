@@ -105,9 +105,14 @@ int bfvmb_sync(bfvmb_ctx *ctx);
  * sweep; enqueued `count` times on the context's stream. */
 int bfvmb_launch_phase(bfvmb_ctx *ctx, int phase, int count);
 
-/* Launch shape of the sweep kernels (0 = library default): resident blocks for the y/z
- * sweeps (persistent grid) and for the x sweep.  Used by the autotuner. */
-int bfvmb_set_launch(bfvmb_ctx *ctx, int yz_blocks, int x_blocks);
+/* Launch shape of the sweep kernels (0 = library default): grid size of the y/z and of the
+ * x sweep (a smaller grid strides over the line tiles), and how many sweep wavefronts
+ * share one CU's LDS line store (1 = the whole LDS per wavefront).  Used by the autotuner;
+ * results do not depend on it. */
+int bfvmb_set_launch(bfvmb_ctx *ctx, int yz_blocks, int x_blocks, int lds_blocks_per_cu);
+
+/* LDS bytes per workgroup and CU count of HIP device `device`. */
+int bfvmb_device_info(int device, int *lds_per_block, int *cu_count);
 
 /* Device address of the current field, [nsub][nz][ny][nx] of 8- or 4-byte reals. */
 int bfvmb_density_device_ptr(bfvmb_ctx *ctx, void **ptr);

@@ -47,6 +47,9 @@ CASES = {
     "one_voxel": dict(nx=1, ny=1, nz=1, nsub=3, seed=5, nagents=4, nsteps=2),
     "chunks_130x70x33x2": dict(nx=130, ny=70, nz=33, nsub=2, seed=6, nagents=3000, nsteps=6, shared_voxels=True),
     "long_x_300x17x19x1": dict(nx=300, ny=17, nz=19, nsub=1, seed=7, nagents=100, nsteps=4),
+    "long_x_700x5x6x2": dict(nx=700, ny=5, nz=6, nsub=2, seed=15, nagents=50, nsteps=3),
+    "long_y_9x340x7x1": dict(nx=9, ny=340, nz=7, nsub=1, seed=16, nagents=50, nsteps=3),
+    "long_z_5x6x700x2": dict(nx=5, ny=6, nz=700, nsub=2, seed=17, nagents=50, nsteps=3),
     "all_dirichlet_64x48x40x4": dict(nx=64, ny=48, nz=40, nsub=4, seed=8, nagents=2000, nsteps=8, dactive=[1, 1, 1, 1]),
     "no_dirichlet_40x41x42x2": dict(nx=40, ny=41, nz=42, nsub=2, seed=9, nagents=500, nsteps=8, dactive=[0, 0]),
     "config1_50cube_100steps": dict(nx=50, ny=50, nz=50, nsub=1, seed=10, nagents=1000, nsteps=100, D=[1e5], decay=[0.1],
@@ -119,13 +122,21 @@ def test_set_agents_with_temps_and_inactive():
     s.close()
 
 
-def test_launch_shapes_do_not_change_results():
-    """Bounded persistent grids give the same bits as one block per tile."""
-    case = _case("chunks_130x70x33x2")
-    ref = gpu_run(case, "fp64")
-    s = _solver(case, "fp64")
-    s.set_launch(7, 5)
-    s.run(case.nsteps)
-    got = s.download()
-    s.close()
-    assert np.array_equal(got, ref)
+@pytest.mark.parametrize("name", ["chunks_130x70x33x2", "long_x_700x5x6x2", "long_z_5x6x700x2"])
+@pytest.mark.parametrize("launch", [(7, 5, 1), (0, 0, 2), (3, 11, 8)])
+def test_launch_shapes_do_not_change_results(name, launch):
+    """Strided grids and smaller LDS line stores (more of each line parked in the output
+    buffer) give the same bits as the default launch."""
+    case = _case(name)
+    print(f"[gpu] launch {launch} {name}", flush=True)
+    ref = orc.run(case)
+    for prec in ("fp64", "fp32"):
+        s = _solver(case, prec)
+        s.set_launch(*launch)
+        s.run(case.nsteps)
+        got = s.download()
+        s.close()
+        if prec == "fp64":
+            assert np.array_equal(got, ref)
+        else:
+            assert rel_err(got, ref)[0] <= 1e-5
