@@ -36,7 +36,7 @@ SIGNATURES = {
     "bfvmb_download_density": (_c.c_int, [_vp, _dp]),
     "bfvmb_set_agents": (_c.c_int, [_vp, _c.c_int, _ip, _dp, _dp, _bp]),
     "bfvmb_set_agent_rates": (_c.c_int, [_vp, _c.c_int, _ip, _dp, _dp, _dp, _dp, _bp, _c.c_double]),
-    "bfvmb_voxel_indices": (_c.c_int, [_vp, _c.c_int, _dp, _dp, _ip]),
+    "bfvmb_voxel_indices": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int, _c.c_double, _dp, _c.c_int, _dp, _ip]),
     "bfvmb_simulate_cell_sources_and_sinks": (_c.c_int, [_vp]),
     "bfvmb_simulate_diffusion_decay": (_c.c_int, [_vp, _c.c_int]),
     "bfvmb_run": (_c.c_int, [_vp, _c.c_int]),

@@ -16,6 +16,20 @@ from . import _lib as L
 PRECISIONS = {"fp64": L.FP64, "fp32": L.FP32}
 
 
+def voxel_indices(grid, dx, pos, bbox_min=(0.0, 0.0, 0.0)) -> np.ndarray:
+    """Voxel of each agent position (bfvmb_voxel_indices, host only): floor((p-min)/dx)
+    clamped into the grid, -1 outside the bounding box."""
+    lib = L.load()
+    P = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    lo = np.ascontiguousarray(bbox_min, dtype=np.float64)
+    out = np.empty(P.shape[0], dtype=np.int32)
+    nx, ny, nz = (int(v) for v in grid)
+    rc = lib.bfvmb_voxel_indices(nx, ny, nz, float(dx), L.dptr(lo), P.shape[0], L.dptr(P), L.iptr(out))
+    if rc != L.OK:
+        raise L.BioFVMBError(f"bfvmb_voxel_indices failed ({rc})")
+    return out
+
+
 class LODSolver:
     def __init__(self, nx: int, ny: int, nz: int, nsub: int, dx: float, precision: str = "fp64",
                  device: int = 0):
@@ -100,12 +114,7 @@ class LODSolver:
 
     def voxel_indices(self, pos, bbox_min=(0.0, 0.0, 0.0)) -> np.ndarray:
         """Agent voxel per position (-1 outside the bounding box), reference numbering."""
-        P = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
-        lo = np.ascontiguousarray(bbox_min, dtype=np.float64)
-        out = np.empty(P.shape[0], dtype=np.int32)
-        self._check(self._lib.bfvmb_voxel_indices(self._h, P.shape[0], L.dptr(P), L.dptr(lo), L.iptr(out)),
-                    "voxel_indices")
-        return out
+        return voxel_indices(self.shape[:3], self.dx, pos, bbox_min)
 
     def set_launch(self, yz_blocks: int = 0, x_blocks: int = 0, lds_blocks_per_cu: int = 0) -> None:
         self._check(self._lib.bfvmb_set_launch(self._h, yz_blocks, x_blocks, lds_blocks_per_cu), "set_launch")

@@ -367,12 +367,12 @@ int bfvmb_set_agent_rates(bfvmb_ctx *ctx, int n, const int *voxel, const double 
     return bfvmb_set_agents(ctx, n, voxel, t1.data(), t2.data(), active);
 }
 
-int bfvmb_voxel_indices(const bfvmb_ctx *ctx, int n, const double *pos, const double *bbox_min, int *voxel)
+int bfvmb_voxel_indices(int nx, int ny, int nz, double dx, const double *bbox_min, int n, const double *pos,
+                        int *voxel)
 {
-    if (!ctx || n < 0 || (n > 0 && (!pos || !bbox_min || !voxel)))
+    if (nx < 1 || ny < 1 || nz < 1 || !(dx > 0.0) || n < 0 || !bbox_min || (n > 0 && (!pos || !voxel)))
         return BFVMB_ERR_ARG;
-    const double dx = ctx->dx;
-    const int nn[3] = {ctx->nx, ctx->ny, ctx->nz};
+    const int nn[3] = {nx, ny, nz};
     double lo[3], hi[3];
     int g[3];
     for (int a = 0; a < 3; ++a) {
@@ -380,7 +380,7 @@ int bfvmb_voxel_indices(const bfvmb_ctx *ctx, int n, const double *pos, const do
         hi[a] = bbox_min[a] + nn[a] * dx;
         g[a] = (int)((hi[a] - lo[a]) / dx);  // voxels per axis as the reference counts them
     }
-    const long long nv = (long long)ctx->nvox;
+    const long long nv = (long long)nx * ny * nz;
     for (int i = 0; i < n; ++i) {
         const double *p = pos + 3 * (size_t)i;
         bool inside = true;

@@ -1,0 +1,178 @@
+// bfvmb_driver.cpp -- runs one case through the C++ host layer on the GPU.
+//
+// Reads the same case files as the reference driver used for the golden vectors and
+// writes the same output format, so the two are compared directly.  The set-up and the
+// time loop follow the reference's demo drivers (examples/complete_diffusion_demo.cpp:
+// 223-240, 430-493; examples/test_VS.cpp:215-291): resize_densities + set_density,
+// resize_space_uniform, Dirichlet per substrate, initial field, agents through
+// create_basic_agent / assign_position / set_total_volume / rates /
+// set_internal_uptake_constants, then per step simulate_cell_sources_and_sinks(dt)
+// followed by simulate_diffusion_decay(dt).
+//
+// usage: bfvmb_driver <case.bin> <out.bin> [fp64|fp32] [--fused] [--time]
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "biofvmb_host.hpp"
+
+namespace {
+template <class T>
+void get(FILE *f, T *p, size_t n)
+{
+    if (n && std::fread(p, sizeof(T), n, f) != n) {
+        std::fprintf(stderr, "bfvmb_driver: short read\n");
+        std::exit(2);
+    }
+}
+template <class T>
+T one(FILE *f)
+{
+    T v;
+    get(f, &v, 1);
+    return v;
+}
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    if (argc < 3) {
+        std::fprintf(stderr, "usage: %s case.bin out.bin [fp64|fp32] [--fused] [--time]\n", argv[0]);
+        return 2;
+    }
+    int precision = BFVMB_FP64;
+    bool fused = false, timing = false;
+    for (int i = 3; i < argc; ++i) {
+        if (!std::strcmp(argv[i], "fp32"))
+            precision = BFVMB_FP32;
+        else if (!std::strcmp(argv[i], "--fused"))
+            fused = true;
+        else if (!std::strcmp(argv[i], "--time"))
+            timing = true;
+    }
+    FILE *f = std::fopen(argv[1], "rb");
+    if (!f) {
+        std::perror(argv[1]);
+        return 2;
+    }
+    char magic[8];
+    get(f, magic, 8);
+    if (std::memcmp(magic, "BFVMCAS1", 8) != 0) {
+        std::fprintf(stderr, "bad case magic\n");
+        return 2;
+    }
+    const int nx = one<int>(f), ny = one<int>(f), nz = one<int>(f), nsub = one<int>(f);
+    const int nsteps = one<int>(f), with_agents = one<int>(f), nagents = one<int>(f), nsnap = one<int>(f);
+    const double x0 = one<double>(f), y0 = one<double>(f), z0 = one<double>(f), dx = one<double>(f), dt = one<double>(f);
+    std::vector<double> D(nsub), decay(nsub), dval(nsub);
+    std::vector<int> dact(nsub);
+    get(f, D.data(), nsub);
+    get(f, decay.data(), nsub);
+    get(f, dval.data(), nsub);
+    get(f, dact.data(), nsub);
+    const size_t nvox = (size_t)nx * ny * nz;
+    std::vector<double> init(nvox * nsub);
+    get(f, init.data(), init.size());
+    std::vector<double> pos(3 * (size_t)nagents), vol(nagents), S((size_t)nagents * nsub), U((size_t)nagents * nsub),
+        T((size_t)nagents * nsub);
+    get(f, pos.data(), pos.size());
+    get(f, vol.data(), vol.size());
+    get(f, S.data(), S.size());
+    get(f, U.data(), U.size());
+    get(f, T.data(), T.size());
+    std::vector<int> snaps(nsnap);
+    get(f, snaps.data(), nsnap);
+    std::fclose(f);
+
+    try {
+        biofvmb::Microenvironment M(precision);
+        M.resize_densities(nsub);
+        for (int d = 0; d < nsub; ++d)
+            M.set_density(d, "substrate" + std::to_string(d), "dimensionless", D[d], decay[d]);
+        M.resize_space_uniform(x0, x0 + nx * dx, y0, y0 + ny * dx, z0, z0 + nz * dx, dx);
+        if (M.x_size() != nx || M.y_size() != ny || M.z_size() != nz) {
+            std::fprintf(stderr, "mesh size mismatch\n");
+            return 3;
+        }
+        for (int d = 0; d < nsub; ++d) {
+            if (dact[d] < 0)
+                continue;  // keep resize_space's defaults (100.0, active)
+            M.set_substrate_dirichlet_value(d, dval[d]);
+            M.set_substrate_dirichlet_activation(d, dact[d] != 0);
+        }
+        std::copy(init.begin(), init.end(), M.density_data());
+
+        std::vector<int> voxel(nagents);
+        std::vector<double> t1((size_t)nagents * nsub), t2((size_t)nagents * nsub);
+        for (int a = 0; a < nagents; ++a) {
+            biofvmb::Basic_Agent *ag = biofvmb::create_basic_agent();
+            ag->register_microenvironment(&M);
+            ag->assign_position(pos[3 * a], pos[3 * a + 1], pos[3 * a + 2]);
+            ag->set_total_volume(vol[a]);
+            for (int d = 0; d < nsub; ++d) {
+                ag->secretion_rates[d] = S[(size_t)a * nsub + d];
+                ag->uptake_rates[d] = U[(size_t)a * nsub + d];
+                ag->saturation_densities[d] = T[(size_t)a * nsub + d];
+            }
+            ag->set_internal_uptake_constants(dt);
+            voxel[a] = ag->get_current_voxel_index();
+            for (int d = 0; d < nsub; ++d) {
+                t1[(size_t)a * nsub + d] = ag->temp1()[d];
+                t2[(size_t)a * nsub + d] = ag->temp2()[d];
+            }
+        }
+
+        FILE *out = std::fopen(argv[2], "wb");
+        if (!out) {
+            std::perror(argv[2]);
+            return 2;
+        }
+        std::fwrite("BFVMOUT1", 1, 8, out);
+        std::fwrite(&nagents, sizeof(int), 1, out);
+        std::fwrite(voxel.data(), sizeof(int), nagents, out);
+        std::fwrite(t1.data(), sizeof(double), t1.size(), out);
+        std::fwrite(t2.data(), sizeof(double), t2.size(), out);
+        std::fwrite(&nsnap, sizeof(int), 1, out);
+        int next = 0, done = 0;
+        double ms = 0.0;
+        for (int s = 1; s <= nsteps; ++s) {
+            const bool snap = next < nsnap && snaps[next] == s;
+            if (fused && !snap && s < nsteps)
+                continue;  // batch steps up to the next snapshot into one call
+            const int k = s - done;
+            auto ta = std::chrono::steady_clock::now();
+            if (fused) {
+                if (with_agents)
+                    M.run(biofvmb::all_basic_agents, dt, k);
+                else
+                    for (int i = 0; i < k; ++i)
+                        M.simulate_diffusion_decay(dt);
+            } else {
+                for (int i = 0; i < k; ++i) {
+                    if (with_agents)
+                        M.simulate_cell_sources_and_sinks(dt);
+                    M.simulate_diffusion_decay(dt);
+                }
+            }
+            bfvmb_sync(M.context());
+            ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ta).count();
+            done = s;
+            while (next < nsnap && snaps[next] == s) {
+                std::fwrite(&s, sizeof(int), 1, out);
+                std::fwrite(M.density_data(), sizeof(double), nvox * nsub, out);
+                ++next;
+            }
+        }
+        std::fclose(out);
+        if (timing)
+            std::printf("{\"steps\": %d, \"ms_total\": %.6f}\n", nsteps, ms);
+        biofvmb::delete_all_basic_agents();
+    } catch (const biofvmb::Error &e) {
+        std::fprintf(stderr, "bfvmb_driver: %s (code %d)\n", e.what(), e.code);
+        return 4;
+    }
+    return 0;
+}

@@ -1,0 +1,27 @@
+// bfvmb_reference_plugin.hpp -- drop-in entry points for an application built on the
+// reference BioFVM-B library.  Include after the application's own "BioFVM.h".
+//
+//   M.diffusion_decay_solver = bfvmb_hip_LOD_3D;        // replaces
+//       diffusion_decay_solver__constant_coefficients_LOD_3D (BioFVM_parallel.cpp:693),
+//       same slot signature (BioFVM_microenvironment.h:164)
+//   bfvmb_hip_cell_sources_and_sinks(M, all_basic_agents, dt);   // replaces
+//       Microenvironment::simulate_cell_sources_and_sinks (BioFVM_microenvironment.cpp:1155-1169)
+//
+// Both run on one MI355X through the C ABI of include/biofvmb.h.  By default every call
+// uploads p_density_vectors, runs, and downloads the result, so the host field is always
+// current (exactly the reference's contract).  bfvmb_plugin_set_resident(M, true) keeps the
+// field on the device between calls instead; bfvmb_plugin_sync(M) then brings the host
+// copy up to date and bfvmb_plugin_invalidate(M) uploads host-side edits on the next call.
+#pragma once
+
+#include <vector>
+
+void bfvmb_hip_LOD_3D(BioFVM::Microenvironment &M, double dt, int mpi_Size, int mpi_Rank, int *mpi_Coords,
+                      int *mpi_Dims, MPI_Comm mpi_Cart_comm);
+void bfvmb_hip_cell_sources_and_sinks(BioFVM::Microenvironment &M, std::vector<BioFVM::Basic_Agent *> &agents,
+                                      double dt);
+
+void bfvmb_plugin_set_resident(BioFVM::Microenvironment &M, bool resident);
+void bfvmb_plugin_sync(BioFVM::Microenvironment &M);        // device -> p_density_vectors
+void bfvmb_plugin_invalidate(BioFVM::Microenvironment &M);  // host field edited: upload next call
+void bfvmb_plugin_release(BioFVM::Microenvironment &M);     // free the device context

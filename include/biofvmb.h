@@ -84,8 +84,8 @@ int bfvmb_set_agent_rates(bfvmb_ctx *ctx, int n, const int *voxel, const double 
  * (BioFVM_basic_agent.cpp:112-153, BioFVM_parallel.cpp:418-522 at world size 1):
  * floor((p - min)/dx) clamped into the grid.  A position outside the bounding box
  * (BioFVM_mesh.cpp:204-213) gets -1; the reference then leaves that agent at voxel 0. */
-int bfvmb_voxel_indices(const bfvmb_ctx *ctx, int n, const double *pos, const double *bbox_min,
-                        int *voxel);
+int bfvmb_voxel_indices(int nx, int ny, int nz, double dx, const double *bbox_min, int n,
+                        const double *pos, int *voxel);
 
 /* Microenvironment::simulate_cell_sources_and_sinks (BioFVM_microenvironment.cpp:1155-1169). */
 int bfvmb_simulate_cell_sources_and_sinks(bfvmb_ctx *ctx);

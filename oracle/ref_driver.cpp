@@ -13,6 +13,8 @@
 //   simulate_cell_sources_and_sinks(dt) followed by simulate_diffusion_decay(...).
 //
 // usage: ref_driver <case.bin> <out.bin> [lod|avx] [granularity] [--time]
+//        ref_driver_hip <case.bin> <out.bin> [hip|hipres] ...   (BFVMB_PLUGIN build: the
+//        GPU solver in the diffusion_decay_solver slot; hipres keeps the field resident)
 #include <mpi.h>
 #include <omp.h>
 
@@ -24,6 +26,10 @@
 #include <vector>
 
 #include "BioFVM.h"
+#ifdef BFVMB_PLUGIN
+// the same driver with the MI355X entry points plugged into the reference's slots
+#include "bfvmb_reference_plugin.hpp"
+#endif
 
 using namespace BioFVM;
 
@@ -111,6 +117,16 @@ int main(int argc, char **argv)
     slot_t lod = diffusion_decay_solver__constant_coefficients_LOD_3D;
     slot_t avx = diffusion_decay_solver__constant_coefficients_LOD_3D_AVX256D;
     M.diffusion_decay_solver = solver == "avx" ? avx : lod;
+    const bool hip = solver == "hip" || solver == "hipres";
+#ifdef BFVMB_PLUGIN
+    if (hip)
+        M.diffusion_decay_solver = bfvmb_hip_LOD_3D;
+#else
+    if (hip) {
+        fprintf(stderr, "this build has no GPU plugin\n");
+        return 2;
+    }
+#endif
     M.granurality = gran;
     std::copy(init.begin(), init.end(), M.p_density_vectors->begin());
 
@@ -149,13 +165,27 @@ int main(int argc, char **argv)
     double total_ms = 0.0;
     for (int s = 1; s <= nsteps; ++s) {
         auto t_a = std::chrono::steady_clock::now();
-        if (with_agents)
-            M.simulate_cell_sources_and_sinks(dt);
+        if (with_agents) {
+#ifdef BFVMB_PLUGIN
+            if (hip)
+                bfvmb_hip_cell_sources_and_sinks(M, all_basic_agents, dt);
+            else
+#endif
+                M.simulate_cell_sources_and_sinks(dt);
+        }
+#ifdef BFVMB_PLUGIN
+        if (solver == "hipres" && s == 1)
+            bfvmb_plugin_set_resident(M, true);
+#endif
         M.simulate_diffusion_decay(dt, size, rank, coords, dims, cart);
         auto t_b = std::chrono::steady_clock::now();
         total_ms += std::chrono::duration<double, std::milli>(t_b - t_a).count();
         while (next < nsnap && snaps[next] == s) {
             fwrite(&s, sizeof(int), 1, out);
+#ifdef BFVMB_PLUGIN
+            if (hip)
+                bfvmb_plugin_sync(M);
+#endif
             fwrite(M.p_density_vectors->data(), sizeof(double), nvox * nsub, out);
             ++next;
         }

@@ -56,4 +56,18 @@ def test_null_context_is_an_argument_error():
     assert lib.bfvmb_destroy(None) == L.OK
     v = np.zeros(1, np.int32)
     p = np.zeros(3)
-    assert lib.bfvmb_voxel_indices(None, 1, L.dptr(p), L.dptr(p), L.iptr(v)) == L.ERR_ARG
+    assert lib.bfvmb_voxel_indices(0, 4, 4, 20.0, L.dptr(p), 1, L.dptr(p), L.iptr(v)) == L.ERR_ARG
+
+
+def test_voxel_mapping_matches_reference_fixtures():
+    """Host-side agent -> voxel mapping against the reference's own assign_position
+    results (out-of-box agents: -1 here, voxel 0 in the reference)."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    import make_golden as golden
+    from biofvmb.solver import voxel_indices
+    for name in golden.NAMES:
+        case, ref = golden.load(name)
+        got = voxel_indices((case.nx, case.ny, case.nz), case.dx, case.pos, case.bbox[:3])
+        assert np.array_equal(np.where(got < 0, 0, got), ref["voxel"]), name

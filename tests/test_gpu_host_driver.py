@@ -1,0 +1,38 @@
+"""The C++ host layer (biofvm-b_amd/host: Microenvironment / Basic_Agent over the C ABI),
+run as a driver on the reference's own case files, against the reference's outputs:
+agent mapping, source/sink constants and every snapshot must match bit for bit."""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+import casefile as cf
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+import make_golden as golden  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "biofvm-b_amd", "lib", "bfvmb_driver")
+
+
+@pytest.mark.parametrize("mode", ["per_call", "fused"])
+@pytest.mark.parametrize("name", golden.NAMES)
+def test_host_driver_matches_reference(name, mode):
+    case, ref = golden.load(name)
+    print(f"[gpu] host driver {name} {mode}", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        cpath, opath = os.path.join(d, "case.bin"), os.path.join(d, "out.bin")
+        case.write(cpath)
+        args = [DRIVER, cpath, opath, "fp64"] + (["--fused"] if mode == "fused" else [])
+        subprocess.run(args, check=True, timeout=120)
+        out = cf.read_output(opath, case)
+    assert np.array_equal(out["voxel"], ref["voxel"])
+    assert np.array_equal(out["temp1"], ref["temp1"])
+    assert np.array_equal(out["temp2"], ref["temp2"])
+    for step, f in ref["fields"].items():
+        assert np.array_equal(out["fields"][step], f), f"{name} step {step}: {(out['fields'][step] != f).sum()} differ"

@@ -1,0 +1,39 @@
+"""Drop-in check: the reference's own library and driver (compiled from its sources by
+oracle/Makefile, target ref-hip) with bfvmb_hip_LOD_3D in the diffusion_decay_solver slot
+and bfvmb_hip_cell_sources_and_sinks in place of simulate_cell_sources_and_sinks
+(biofvm-b_amd/host/bfvmb_reference_plugin.cpp) must reproduce the reference solver's
+outputs bit for bit, both with the field synced every call and kept resident."""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+import casefile as cf
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+import make_golden as golden  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver_hip")
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="reference build with the plugin (make -C oracle ref-hip) absent")
+@pytest.mark.parametrize("mode", ["hip", "hipres"])
+@pytest.mark.parametrize("name", golden.NAMES)
+def test_reference_driver_with_gpu_slot(name, mode):
+    case, ref = golden.load(name)
+    print(f"[gpu] reference driver + plugin {name} {mode}", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        cpath, opath = os.path.join(d, "case.bin"), os.path.join(d, "out.bin")
+        case.write(cpath)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([DRIVER, cpath, opath, mode, "1"], check=True, timeout=120, env=env)
+        out = cf.read_output(opath, case)
+    assert np.array_equal(out["voxel"], ref["voxel"])
+    for step, f in ref["fields"].items():
+        assert np.array_equal(out["fields"][step], f), f"{name} step {step}: {(out['fields'][step] != f).sum()} differ"
