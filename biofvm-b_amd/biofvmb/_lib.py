@@ -42,9 +42,24 @@ SIGNATURES = {
     "bfvmb_run": (_c.c_int, [_vp, _c.c_int]),
     "bfvmb_sync": (_c.c_int, [_vp]),
     "bfvmb_launch_phase": (_c.c_int, [_vp, _c.c_int, _c.c_int]),
-    "bfvmb_set_launch": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_int]),
+    "bfvmb_set_launch": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_int, _c.c_int]),
+    "bfvmb_run_graph": (_c.c_int, [_vp, _c.c_int]),
+    "bfvmb_autotune": (_c.c_int, [_vp, _c.c_int, _ip]),
     "bfvmb_device_info": (_c.c_int, [_c.c_int, _ip, _ip]),
     "bfvmb_density_device_ptr": (_c.c_int, [_vp, _c.POINTER(_vp)]),
+    "bfvmb_batch_create": (_c.c_int, [_c.POINTER(_vp), _c.c_int, _ip, _ip, _ip, _ip, _dp, _c.c_int, _c.c_int]),
+    "bfvmb_batch_destroy": (_c.c_int, [_vp]),
+    "bfvmb_batch_last_error": (_c.c_char_p, [_vp]),
+    "bfvmb_batch_set_stream": (_c.c_int, [_vp, _vp]),
+    "bfvmb_batch_set_coefficients": (_c.c_int, [_vp, _c.c_int, _dp, _dp, _c.c_double]),
+    "bfvmb_batch_set_dirichlet": (_c.c_int, [_vp, _c.c_int, _dp, _bp]),
+    "bfvmb_batch_upload_density": (_c.c_int, [_vp, _c.c_int, _dp]),
+    "bfvmb_batch_download_density": (_c.c_int, [_vp, _c.c_int, _dp]),
+    "bfvmb_batch_set_agents": (_c.c_int, [_vp, _c.c_int, _c.c_int, _ip, _dp, _dp, _bp]),
+    "bfvmb_batch_run": (_c.c_int, [_vp, _c.c_int, _c.c_int]),
+    "bfvmb_batch_sync": (_c.c_int, [_vp]),
+    "bfvmb_batch_set_launch": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_int, _c.c_int]),
+    "bfvmb_batch_autotune": (_c.c_int, [_vp, _c.c_int, _ip]),
 }
 
 _lib = None

@@ -116,8 +116,9 @@ class LODSolver:
         """Agent voxel per position (-1 outside the bounding box), reference numbering."""
         return voxel_indices(self.shape[:3], self.dx, pos, bbox_min)
 
-    def set_launch(self, yz_blocks: int = 0, x_blocks: int = 0, lds_blocks_per_cu: int = 0) -> None:
-        self._check(self._lib.bfvmb_set_launch(self._h, yz_blocks, x_blocks, lds_blocks_per_cu), "set_launch")
+    def set_launch(self, sweep: int = -1, grid: int = 0, lds_blocks_per_cu: int = 0, ring: int = 0) -> None:
+        """Launch shape of sweep 0 (x), 1 (y), 2 (z) or -1 (all); zeros = library default."""
+        self._check(self._lib.bfvmb_set_launch(self._h, sweep, grid, lds_blocks_per_cu, ring), "set_launch")
 
     # -- stepping -------------------------------------------------------------------------
     def simulate_cell_sources_and_sinks(self) -> None:
@@ -126,8 +127,15 @@ class LODSolver:
     def simulate_diffusion_decay(self, nsteps: int = 1) -> None:
         self._check(self._lib.bfvmb_simulate_diffusion_decay(self._h, nsteps), "simulate_diffusion_decay")
 
-    def run(self, nsteps: int) -> None:
-        self._check(self._lib.bfvmb_run(self._h, nsteps), "run")
+    def run(self, nsteps: int, graph: bool = False) -> None:
+        fn = self._lib.bfvmb_run_graph if graph else self._lib.bfvmb_run
+        self._check(fn(self._h, nsteps), "run")
+
+    def autotune(self, reps: int = 3) -> np.ndarray:
+        """Pick each sweep's launch shape by timing; returns [[wavefronts/CU, ring, us]] x 3."""
+        out = np.zeros(9, dtype=np.int32)
+        self._check(self._lib.bfvmb_autotune(self._h, reps, L.iptr(out)), "autotune")
+        return out.reshape(3, 3)
 
     def launch_phase(self, phase: int, count: int = 1) -> None:
         self._check(self._lib.bfvmb_launch_phase(self._h, phase, count), "launch_phase")
@@ -139,3 +147,87 @@ class LODSolver:
         p = ctypes.c_void_p()
         self._check(self._lib.bfvmb_density_device_ptr(self._h, ctypes.byref(p)), "density_device_ptr")
         return int(p.value or 0)
+
+
+class Batch:
+    """Many independent microenvironments of uneven sizes advanced together
+    (``bfvmb_batch_*``): one kernel launch per phase covers every member, and
+    ``run(..., graph=True)`` replays the whole run as one captured HIP graph."""
+
+    def __init__(self, shapes, dx, precision: str = "fp64", device: int = 0):
+        self._lib = L.load()
+        shapes = [tuple(int(v) for v in s) for s in shapes]
+        n = len(shapes)
+        self.shapes = shapes
+        nx, ny, nz, ns = (np.ascontiguousarray([s[i] for s in shapes], dtype=np.int32) for i in range(4))
+        dxa = np.ascontiguousarray(np.broadcast_to(dx, (n,)), dtype=np.float64)
+        h = ctypes.c_void_p()
+        rc = self._lib.bfvmb_batch_create(ctypes.byref(h), n, L.iptr(nx), L.iptr(ny), L.iptr(nz), L.iptr(ns),
+                                          L.dptr(dxa), PRECISIONS[precision], device)
+        if rc != L.OK:
+            raise L.BioFVMBError(f"bfvmb_batch_create failed ({rc}): {self._lib.bfvmb_batch_last_error(None).decode()}")
+        self._h = h
+        self.dx = dxa
+
+    def _check(self, rc, what):
+        if rc != L.OK:
+            raise L.BioFVMBError(f"{what} failed ({rc}): {self._lib.bfvmb_batch_last_error(self._h).decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.bfvmb_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_ptr):
+        self._check(self._lib.bfvmb_batch_set_stream(self._h, stream_ptr), "set_stream")
+
+    def set_coefficients(self, i, diffusion, decay, dt):
+        D = np.ascontiguousarray(diffusion, dtype=np.float64)
+        lam = np.ascontiguousarray(decay, dtype=np.float64)
+        self._check(self._lib.bfvmb_batch_set_coefficients(self._h, i, L.dptr(D), L.dptr(lam), float(dt)),
+                    "set_coefficients")
+
+    def set_dirichlet(self, i, value, active):
+        ns = self.shapes[i][3]
+        v = np.ascontiguousarray(np.broadcast_to(value, (ns,)), dtype=np.float64)
+        a = np.ascontiguousarray(np.broadcast_to(active, (ns,)), dtype=np.uint8)
+        self._check(self._lib.bfvmb_batch_set_dirichlet(self._h, i, L.dptr(v), L.bptr(a)), "set_dirichlet")
+
+    def upload(self, i, rho):
+        r = np.ascontiguousarray(rho, dtype=np.float64)
+        assert r.size == np.prod(self.shapes[i])
+        self._check(self._lib.bfvmb_batch_upload_density(self._h, i, L.dptr(r)), "upload_density")
+
+    def download(self, i):
+        out = np.empty(self.shapes[i], dtype=np.float64)
+        self._check(self._lib.bfvmb_batch_download_density(self._h, i, L.dptr(out)), "download_density")
+        return out
+
+    def set_agents(self, i, voxel, temp1, temp2, active=None):
+        v = np.ascontiguousarray(voxel, dtype=np.int32)
+        ns = self.shapes[i][3]
+        t1 = np.ascontiguousarray(temp1, dtype=np.float64).reshape(v.size, ns)
+        t2 = np.ascontiguousarray(temp2, dtype=np.float64).reshape(v.size, ns)
+        a = None if active is None else np.ascontiguousarray(active, dtype=np.uint8)
+        self._check(self._lib.bfvmb_batch_set_agents(self._h, i, v.size, L.iptr(v), L.dptr(t1), L.dptr(t2),
+                                                     None if a is None else L.bptr(a)), "set_agents")
+
+    def run(self, nsteps, graph=True):
+        self._check(self._lib.bfvmb_batch_run(self._h, nsteps, 1 if graph else 0), "run")
+
+    def sync(self):
+        self._check(self._lib.bfvmb_batch_sync(self._h), "sync")
+
+    def set_launch(self, sweep=-1, grid=0, lds_blocks_per_cu=0, ring=0):
+        self._check(self._lib.bfvmb_batch_set_launch(self._h, sweep, grid, lds_blocks_per_cu, ring), "set_launch")
+
+    def autotune(self, reps=3):
+        out = np.zeros(9, dtype=np.int32)
+        self._check(self._lib.bfvmb_batch_autotune(self._h, reps, L.iptr(out)), "autotune")
+        return out.reshape(3, 3)

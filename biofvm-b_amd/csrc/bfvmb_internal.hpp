@@ -1,89 +1,115 @@
-// bfvmb_internal.hpp -- context and kernel interfaces shared by the HIP translation units.
+// bfvmb_internal.hpp -- device descriptors, groups of microenvironments and the kernel
+// launchers shared by the HIP translation units.
+//
+// Every context is a Group: one or more microenvironments ("domains") stepped together
+// on one stream, all on the same GPU and in the same precision.  Each phase of a step
+// (sources/sinks, x, y, z sweep) is one kernel launch covering every domain of the group:
+// the kernels read a device array of per-domain descriptors and map a global tile index
+// to its domain through per-phase prefix sums.  A single-domain context (bfvmb_ctx) and a
+// batch of uneven domains (bfvmb_batch) therefore run the same kernels.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "biofvmb.h"
 
 namespace bfvmb {
 
-// Per-axis Thomas tables, FP64 on both storage paths, [nsub][n] (substrate-major so that
-// one substrate's table is contiguous and uniform across a wavefront).
+constexpr int kWave = 64;
+
+// Per-axis Thomas tables, FP64 on both storage paths, [nsub][n].
 struct AxisTables {
     const double *den;   // denominators  (thomas_denom{x,y,z})
     const double *cc;    // upper factors (thomas_c{x,y,z})
     const double *rden;  // RN(1/den)
 };
 
+// Device descriptor of one domain (plain data, copied to the GPU).
 template <class T>
-struct SweepArgs {
-    const T *__restrict__ in;   // field before the sweep   [nsub][nz][ny][nx]
-    T *__restrict__ out;        // field after the sweep (a different buffer)
-    const double *c1;           // [nsub]  D*dt/dx^2
-    AxisTables tab;             // tables of the swept axis
-    const T *dval;              // [nsub]  Dirichlet value
-    const uint8_t *dact;        // [nsub]  Dirichlet activation
+struct DomainDesc {
+    T *field[2];                 // ping-pong buffers [nsub][nz][ny][nx]
+    const double *c1;            // [nsub]  D*dt/dx^2
+    AxisTables tab[3];           // x, y, z tables
+    const T *dval;               // [nsub]  Dirichlet value
+    const uint8_t *dact;         // [nsub]  Dirichlet activation
+    const int *seg_start;        // agents: [nseg+1] offsets into the voxel-sorted list
+    const long long *seg_off;    // agents: [nseg] voxel offset z*nx*ny + y*nx + x
+    const double *t1, *t2;       // agents: [nsub][nact] temp1 / temp2, sorted order
+    int nseg, nact;
     int nx, ny, nz, nsub;
 };
 
-struct AgentCSR {
-    int nseg = 0;                 // voxels holding at least one active agent
-    int nact = 0;                 // active agents
-    int *seg_start = nullptr;     // [nseg+1] offsets into the sorted agent list
-    long long *seg_off = nullptr; // [nseg] voxel offset z*nx*ny + y*nx + x
-    double *t1 = nullptr;         // [nsub][nact] temp1 in sorted order
-    double *t2 = nullptr;         // [nsub][nact] temp2 in sorted order
+// Launch shape of one sweep as set through bfvmb_set_launch (0 = library default).
+struct LaunchShape {
+    long long grid = 0;          // blocks of the (strided) grid
+    int lds_blocks_per_cu = 0;   // sweep wavefronts sharing one CU's LDS line store
+    int ring = 0;                // y/z sweeps: rows in flight per lane (16, 32 or 64)
 };
 
-}  // namespace bfvmb
-
-struct bfvmb_ctx {
+// Host bookkeeping of one domain.
+struct Domain {
     int nx = 0, ny = 0, nz = 0, nsub = 0;
     double dx = 0.0;
-    int precision = BFVMB_FP64;
-    int device = 0;
     size_t nvox = 0;
-    size_t elem = 8;  // bytes per stored real
-    hipStream_t stream = nullptr;
-    hipStream_t own_stream = nullptr;
-
-    void *field[2] = {nullptr, nullptr};  // ping-pong field buffers
-    int cur = 0;                          // which buffer holds the current field
-
-    // coefficient block (FP64): c1[nsub] | per axis den, cc, rden [nsub][n]
-    double *coef = nullptr;
+    void *field[2] = {nullptr, nullptr};
+    double *coef = nullptr;  // c1[nsub] | per axis den, cc, rden [nsub][n]
     bool have_coef = false;
     double dt = 0.0;
     std::vector<double> D, decay;
-
     std::vector<double> dval;
     std::vector<uint8_t> dact;
-    void *d_dval = nullptr;   // [nsub] storage type
+    void *d_dval = nullptr;
     uint8_t *d_dact = nullptr;
+    // agents (device CSR)
+    int nseg = 0, nact = 0;
+    int *seg_start = nullptr;
+    long long *seg_off = nullptr;
+    double *t1 = nullptr, *t2 = nullptr;
+};
 
-    bfvmb::AgentCSR agents;
+// Offsets inside a domain's coefficient block, in doubles.
+struct CoefLayout {
+    size_t c1, den[3], cc[3], rden[3], total;
+};
+CoefLayout coef_layout(const Domain &d);
 
+struct Group {
+    int precision = BFVMB_FP64;
+    int device = 0;
+    size_t elem = 8;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    std::vector<Domain> doms;
+    int cur = 0;  // which buffer of every domain holds the current field
+    LaunchShape launch[3];
+    size_t max_lds_per_block = 0;
+    int cu_count = 0;
+    // device descriptors and per-phase prefix sums (0 = agent work items, 1..3 = x,y,z tiles)
+    void *d_desc = nullptr;
+    long long *d_prefix = nullptr;
+    std::vector<long long> prefix[4];
+    bool desc_dirty = true;
     void *stage = nullptr;  // staging for AoS<->SoA transfers (double)
-    int yz_blocks = 0, x_blocks = 0;
-    int lds_blocks_per_cu = 2;        // sweep wavefronts sharing one CU's LDS
-    size_t max_lds_per_block = 0;     // device limit on LDS per workgroup
-
+    size_t stage_bytes = 0;
+    // captured step graphs, keyed by (steps, parity, with_agents)
+    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
     std::string err;
 };
 
-namespace bfvmb {
-
-// Launchers (lod.hip).
-int launch_sweep(bfvmb_ctx *ctx, int axis);  // one sweep, field[cur] -> field[cur^1]
-// Launchers (field.hip).
-int launch_agents(bfvmb_ctx *ctx);
-int launch_aos_to_soa(bfvmb_ctx *ctx, const double *d_aos, void *d_soa);
-int launch_soa_to_aos(bfvmb_ctx *ctx, const void *d_soa, double *d_aos);
+// Launchers (lod.hip, field.hip): one kernel over all domains of the group.
+int launch_sweep(Group *g, int axis);   // field[cur] -> field[cur^1] for every domain
+int launch_agents(Group *g);            // in place on field[cur]
+int launch_aos_to_soa(Group *g, Domain &d, const double *d_aos, void *d_soa);
+int launch_soa_to_aos(Group *g, Domain &d, const void *d_soa, double *d_aos);
+int sweep_lds_attributes(size_t bytes);
+int sync_descriptors(Group *g);         // upload descriptors/prefixes if stale
 
 // Host FP64 Thomas set-up (thomas.cpp).
 void lod_constants(int nsub, const double *D, const double *decay, double dt, double dx,
@@ -91,16 +117,25 @@ void lod_constants(int nsub, const double *D, const double *decay, double dt, do
 void thomas_axis(int n, int nsub, const double *c1, const double *c2, const double *c3,
                  const double *c3a, double *den, double *cc);
 
-// Opt the sweep kernels into the device's full LDS size (lod.hip).
-int sweep_lds_attributes(size_t bytes);
-
-// Offsets inside the coefficient block, in doubles.
-struct CoefLayout {
-    size_t c1, den[3], cc[3], rden[3], total;
-};
-CoefLayout coef_layout(const bfvmb_ctx *ctx);
-
-int set_error(bfvmb_ctx *ctx, int code, const std::string &msg);
-int hip_check(bfvmb_ctx *ctx, hipError_t e, const char *what);
+// Group operations behind both ABIs (group.hip).
+int set_error(Group *g, int code, const std::string &msg);
+int hip_check(Group *g, hipError_t e, const char *what);
+int group_init(Group *g, int ndom, const int *nx, const int *ny, const int *nz, const int *nsub,
+               const double *dx, int precision, int device);
+void group_free(Group *g);
+int domain_set_coefficients(Group *g, int i, const double *D, const double *decay, double dt);
+int domain_set_dirichlet(Group *g, int i, const double *value, const uint8_t *active);
+int domain_upload(Group *g, int i, const double *rho);
+int domain_download(Group *g, int i, double *rho);
+int domain_set_agents(Group *g, int i, int n, const int *voxel, const double *temp1, const double *temp2,
+                      const uint8_t *active);
+int group_steps(Group *g, int nsteps, bool with_agents, bool use_graph);
+int group_set_stream(Group *g, void *stream);
+int group_launch_phase(Group *g, int phase, int count);
+int group_set_launch(Group *g, int sweep, long long grid, int bpc, int ring);
+int group_autotune(Group *g, int reps, int *chosen);
 
 }  // namespace bfvmb
+
+struct bfvmb_ctx : bfvmb::Group {};
+struct bfvmb_batch : bfvmb::Group {};

@@ -5,29 +5,42 @@
 namespace bfvmb {
 namespace {
 
+__device__ __forceinline__ int find_domain_of(const long long *prefix, int ndom, long long t)
+{
+    int lo = 0, hi = ndom - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
 // Cell sources and sinks (Basic_Agent::simulate_secretion_and_uptake,
 // BioFVM_basic_agent.cpp:279-300, looped by BioFVM_microenvironment.cpp:1155-1169).
-// One thread per (substrate, occupied voxel) applies that voxel's agents in list order:
-//     r = (r + temp1[a]) / temp2[a]
-// The arithmetic is FP64 on both storage paths, so the FP64 path matches the reference
-// bit for bit and the FP32 path rounds once per voxel per step.
+// One thread per (domain, substrate, occupied voxel) applies that voxel's agents in list
+// order:  r = (r + temp1[a]) / temp2[a].  The arithmetic is FP64 on both storage paths, so
+// the FP64 path matches the reference bit for bit and the FP32 path rounds once per voxel.
 template <class T>
-__global__ __launch_bounds__(256) void k_agents(T *rho, const int *__restrict__ seg_start,
-                                                const long long *__restrict__ seg_off,
-                                                const double *__restrict__ t1,
-                                                const double *__restrict__ t2, int nseg, int nsub,
-                                                int nact, size_t nvox)
+__global__ __launch_bounds__(256) void k_agents(const DomainDesc<T> *__restrict__ descs,
+                                                const long long *__restrict__ prefix, int ndom, int parity)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long long)nseg * nsub)
+    if (i >= prefix[ndom])
         return;
-    const int s = (int)(i / nseg);
-    const int k = (int)(i - (long long)s * nseg);
-    T *p = rho + (size_t)s * nvox + seg_off[k];
+    const int di = find_domain_of(prefix, ndom, i);
+    const DomainDesc<T> &D = descs[di];
+    const long long local = i - prefix[di];
+    const int s = (int)(local / D.nseg);
+    const int k = (int)(local - (long long)s * D.nseg);
+    const size_t nvox = (size_t)D.nx * D.ny * D.nz;
+    T *p = D.field[parity] + (size_t)s * nvox + D.seg_off[k];
     double r = (double)*p;
-    const double *a1 = t1 + (size_t)s * nact;
-    const double *a2 = t2 + (size_t)s * nact;
-    for (int a = seg_start[k]; a < seg_start[k + 1]; ++a) {
+    const double *a1 = D.t1 + (size_t)s * D.nact;
+    const double *a2 = D.t2 + (size_t)s * D.nact;
+    for (int a = D.seg_start[k]; a < D.seg_start[k + 1]; ++a) {
         r = r + a1[a];
         r = r / a2[a];
     }
@@ -84,53 +97,55 @@ __global__ __launch_bounds__(256) void k_soa_to_aos(const T *__restrict__ soa, d
 }
 
 template <class T>
-int launch_agents_t(bfvmb_ctx *ctx)
+int launch_agents_t(Group *g)
 {
-    const AgentCSR &A = ctx->agents;
-    if (A.nseg == 0)
+    int rc = sync_descriptors(g);
+    if (rc)
+        return rc;
+    const int ndom = (int)g->doms.size();
+    const long long work = g->prefix[0][ndom];
+    if (work == 0)
         return BFVMB_OK;
-    long long work = (long long)A.nseg * ctx->nsub;
-    unsigned grid = (unsigned)((work + 255) / 256);
-    hipLaunchKernelGGL(k_agents<T>, dim3(grid), dim3(256), 0, ctx->stream,
-                       static_cast<T *>(ctx->field[ctx->cur]), A.seg_start, A.seg_off, A.t1, A.t2,
-                       A.nseg, ctx->nsub, A.nact, ctx->nvox);
-    return hip_check(ctx, hipGetLastError(), "agents launch");
+    const unsigned grid = (unsigned)((work + 255) / 256);
+    hipLaunchKernelGGL(k_agents<T>, dim3(grid), dim3(256), 0, g->stream, static_cast<const DomainDesc<T> *>(g->d_desc),
+                       g->d_prefix, ndom, g->cur);
+    return hip_check(g, hipGetLastError(), "agents launch");
 }
 
-dim3 transpose_grid(const bfvmb_ctx *ctx)
+dim3 transpose_grid(const Domain &d)
 {
-    return dim3((unsigned)((ctx->nz + 31) / 32), (unsigned)((ctx->nx + 31) / 32), (unsigned)ctx->ny);
+    return dim3((unsigned)((d.nz + 31) / 32), (unsigned)((d.nx + 31) / 32), (unsigned)d.ny);
 }
 
 }  // namespace
 
-int launch_agents(bfvmb_ctx *ctx)
+int launch_agents(Group *g)
 {
-    if (ctx->precision == BFVMB_FP64)
-        return launch_agents_t<double>(ctx);
-    return launch_agents_t<float>(ctx);
+    if (g->precision == BFVMB_FP64)
+        return launch_agents_t<double>(g);
+    return launch_agents_t<float>(g);
 }
 
-int launch_aos_to_soa(bfvmb_ctx *ctx, const double *d_aos, void *d_soa)
+int launch_aos_to_soa(Group *g, Domain &d, const double *d_aos, void *d_soa)
 {
-    if (ctx->precision == BFVMB_FP64)
-        hipLaunchKernelGGL(k_aos_to_soa<double>, transpose_grid(ctx), dim3(32, 8), 0, ctx->stream, d_aos,
-                           static_cast<double *>(d_soa), ctx->nx, ctx->ny, ctx->nz, ctx->nsub);
+    if (g->precision == BFVMB_FP64)
+        hipLaunchKernelGGL(k_aos_to_soa<double>, transpose_grid(d), dim3(32, 8), 0, g->stream, d_aos,
+                           static_cast<double *>(d_soa), d.nx, d.ny, d.nz, d.nsub);
     else
-        hipLaunchKernelGGL(k_aos_to_soa<float>, transpose_grid(ctx), dim3(32, 8), 0, ctx->stream, d_aos,
-                           static_cast<float *>(d_soa), ctx->nx, ctx->ny, ctx->nz, ctx->nsub);
-    return hip_check(ctx, hipGetLastError(), "aos->soa launch");
+        hipLaunchKernelGGL(k_aos_to_soa<float>, transpose_grid(d), dim3(32, 8), 0, g->stream, d_aos,
+                           static_cast<float *>(d_soa), d.nx, d.ny, d.nz, d.nsub);
+    return hip_check(g, hipGetLastError(), "aos->soa launch");
 }
 
-int launch_soa_to_aos(bfvmb_ctx *ctx, const void *d_soa, double *d_aos)
+int launch_soa_to_aos(Group *g, Domain &d, const void *d_soa, double *d_aos)
 {
-    if (ctx->precision == BFVMB_FP64)
-        hipLaunchKernelGGL(k_soa_to_aos<double>, transpose_grid(ctx), dim3(32, 8), 0, ctx->stream,
-                           static_cast<const double *>(d_soa), d_aos, ctx->nx, ctx->ny, ctx->nz, ctx->nsub);
+    if (g->precision == BFVMB_FP64)
+        hipLaunchKernelGGL(k_soa_to_aos<double>, transpose_grid(d), dim3(32, 8), 0, g->stream,
+                           static_cast<const double *>(d_soa), d_aos, d.nx, d.ny, d.nz, d.nsub);
     else
-        hipLaunchKernelGGL(k_soa_to_aos<float>, transpose_grid(ctx), dim3(32, 8), 0, ctx->stream,
-                           static_cast<const float *>(d_soa), d_aos, ctx->nx, ctx->ny, ctx->nz, ctx->nsub);
-    return hip_check(ctx, hipGetLastError(), "soa->aos launch");
+        hipLaunchKernelGGL(k_soa_to_aos<float>, transpose_grid(d), dim3(32, 8), 0, g->stream,
+                           static_cast<const float *>(d_soa), d_aos, d.nx, d.ny, d.nz, d.nsub);
+    return hip_check(g, hipGetLastError(), "soa->aos launch");
 }
 
 }  // namespace bfvmb

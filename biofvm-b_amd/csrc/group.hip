@@ -1,0 +1,572 @@
+// group.hip -- operations on a group of microenvironments: device state, Thomas set-up,
+// transfers, agents, stepping (eager or as a captured HIP graph) and the autotuner.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "bfvmb_internal.hpp"
+
+namespace bfvmb {
+
+int set_error(Group *g, int code, const std::string &msg)
+{
+    if (g)
+        g->err = msg;
+    return code;
+}
+
+int hip_check(Group *g, hipError_t e, const char *what)
+{
+    if (e == hipSuccess)
+        return BFVMB_OK;
+    return set_error(g, e == hipErrorOutOfMemory ? BFVMB_ERR_NOMEM : BFVMB_ERR_HIP,
+                     std::string(what) + ": " + hipGetErrorString(e));
+}
+
+CoefLayout coef_layout(const Domain &d)
+{
+    CoefLayout L;
+    const size_t s = (size_t)d.nsub;
+    const int n[3] = {d.nx, d.ny, d.nz};
+    size_t off = 0;
+    L.c1 = off;
+    off += s;
+    for (int a = 0; a < 3; ++a) {
+        L.den[a] = off;
+        off += s * n[a];
+        L.cc[a] = off;
+        off += s * n[a];
+        L.rden[a] = off;
+        off += s * n[a];
+    }
+    L.total = off;
+    return L;
+}
+
+namespace {
+
+void clear_graphs(Group *g)
+{
+    for (auto &kv : g->graphs)
+        (void)hipGraphExecDestroy(kv.second);
+    g->graphs.clear();
+}
+
+int upload_dirichlet(Group *g, Domain &d)
+{
+    std::vector<unsigned char> vals(d.nsub * g->elem);
+    for (int s = 0; s < d.nsub; ++s) {
+        if (g->elem == 8) {
+            const double v = d.dval[s];
+            std::memcpy(vals.data() + s * 8, &v, 8);
+        } else {
+            const float v = (float)d.dval[s];
+            std::memcpy(vals.data() + s * 4, &v, 4);
+        }
+    }
+    int rc = hip_check(g, hipMemcpyAsync(d.d_dval, vals.data(), vals.size(), hipMemcpyHostToDevice, g->stream),
+                       "dirichlet values");
+    if (!rc)
+        rc = hip_check(g, hipMemcpyAsync(d.d_dact, d.dact.data(), d.nsub, hipMemcpyHostToDevice, g->stream),
+                       "dirichlet flags");
+    if (!rc)
+        rc = hip_check(g, hipStreamSynchronize(g->stream), "dirichlet sync");
+    return rc;
+}
+
+void free_agents(Domain &d)
+{
+    (void)hipFree(d.seg_start);
+    (void)hipFree(d.seg_off);
+    (void)hipFree(d.t1);
+    (void)hipFree(d.t2);
+    d.seg_start = nullptr;
+    d.seg_off = nullptr;
+    d.t1 = d.t2 = nullptr;
+    d.nseg = d.nact = 0;
+}
+
+template <class T>
+std::vector<DomainDesc<T>> build_descs(Group *g)
+{
+    std::vector<DomainDesc<T>> v(g->doms.size());
+    for (size_t i = 0; i < g->doms.size(); ++i) {
+        Domain &d = g->doms[i];
+        DomainDesc<T> &e = v[i];
+        std::memset(&e, 0, sizeof(e));
+        e.field[0] = static_cast<T *>(d.field[0]);
+        e.field[1] = static_cast<T *>(d.field[1]);
+        if (d.coef) {
+            const CoefLayout L = coef_layout(d);
+            e.c1 = d.coef + L.c1;
+            for (int a = 0; a < 3; ++a)
+                e.tab[a] = AxisTables{d.coef + L.den[a], d.coef + L.cc[a], d.coef + L.rden[a]};
+        }
+        e.dval = static_cast<const T *>(d.d_dval);
+        e.dact = d.d_dact;
+        e.seg_start = d.seg_start;
+        e.seg_off = d.seg_off;
+        e.t1 = d.t1;
+        e.t2 = d.t2;
+        e.nseg = d.nseg;
+        e.nact = d.nact;
+        e.nx = d.nx;
+        e.ny = d.ny;
+        e.nz = d.nz;
+        e.nsub = d.nsub;
+    }
+    return v;
+}
+
+}  // namespace
+
+int sync_descriptors(Group *g)
+{
+    if (!g->desc_dirty)
+        return BFVMB_OK;
+    const size_t nd = g->doms.size();
+    for (int p = 0; p < 4; ++p)
+        g->prefix[p].assign(nd + 1, 0);
+    for (size_t i = 0; i < nd; ++i) {
+        const Domain &d = g->doms[i];
+        const long long tx = (long long)((d.ny * (long long)d.nz + kWave - 1) / kWave) * d.nsub;
+        const long long ty = (long long)((d.nx * (long long)d.nz + kWave - 1) / kWave) * d.nsub;
+        const long long tz = (long long)((d.nx * (long long)d.ny + kWave - 1) / kWave) * d.nsub;
+        g->prefix[0][i + 1] = g->prefix[0][i] + (long long)d.nseg * d.nsub;
+        g->prefix[1][i + 1] = g->prefix[1][i] + tx;
+        g->prefix[2][i + 1] = g->prefix[2][i] + ty;
+        g->prefix[3][i + 1] = g->prefix[3][i] + tz;
+    }
+    std::vector<long long> flat;
+    for (int p = 0; p < 4; ++p)
+        flat.insert(flat.end(), g->prefix[p].begin(), g->prefix[p].end());
+    int rc;
+    const size_t desc_bytes = nd * (g->precision == BFVMB_FP64 ? sizeof(DomainDesc<double>) : sizeof(DomainDesc<float>));
+    if (!g->d_desc && (rc = hip_check(g, hipMalloc(&g->d_desc, desc_bytes), "descriptors")))
+        return rc;
+    if (!g->d_prefix && (rc = hip_check(g, hipMalloc((void **)&g->d_prefix, flat.size() * sizeof(long long)), "prefixes")))
+        return rc;
+    if (g->precision == BFVMB_FP64) {
+        auto v = build_descs<double>(g);
+        rc = hip_check(g, hipMemcpyAsync(g->d_desc, v.data(), desc_bytes, hipMemcpyHostToDevice, g->stream), "descriptors");
+    } else {
+        auto v = build_descs<float>(g);
+        rc = hip_check(g, hipMemcpyAsync(g->d_desc, v.data(), desc_bytes, hipMemcpyHostToDevice, g->stream), "descriptors");
+    }
+    if (!rc)
+        rc = hip_check(g, hipMemcpyAsync(g->d_prefix, flat.data(), flat.size() * sizeof(long long), hipMemcpyHostToDevice,
+                                         g->stream), "prefixes");
+    if (!rc)
+        rc = hip_check(g, hipStreamSynchronize(g->stream), "descriptor sync");
+    if (rc)
+        return rc;
+    g->desc_dirty = false;
+    clear_graphs(g);  // captured launches hold the old pointers
+    return BFVMB_OK;
+}
+
+int group_init(Group *g, int ndom, const int *nx, const int *ny, const int *nz, const int *nsub, const double *dx,
+               int precision, int device)
+{
+    if (ndom < 1 || !nx || !ny || !nz || !nsub || !dx || (precision != BFVMB_FP64 && precision != BFVMB_FP32))
+        return set_error(g, BFVMB_ERR_ARG, "bad group arguments");
+    for (int i = 0; i < ndom; ++i) {
+        if (nx[i] < 1 || ny[i] < 1 || nz[i] < 1 || nsub[i] < 1 || !(dx[i] > 0.0))
+            return set_error(g, BFVMB_ERR_ARG, "bad grid, substrate count or dx");
+        if ((long long)ny[i] * nz[i] >= (1LL << 31) || (long long)nx[i] * ny[i] >= (1LL << 31) ||
+            (long long)nx[i] * nz[i] >= (1LL << 31))
+            return set_error(g, BFVMB_ERR_ARG, "plane too large");
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0 || device < 0 || device >= ndev)
+        return set_error(g, BFVMB_ERR_NODEV, "no HIP device");
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess || std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return set_error(g, BFVMB_ERR_NODEV, std::string("device is not gfx950: ") + prop.gcnArchName);
+    g->precision = precision;
+    g->device = device;
+    g->elem = precision == BFVMB_FP64 ? 8 : 4;
+    g->max_lds_per_block = prop.sharedMemPerBlock;
+    g->cu_count = prop.multiProcessorCount;
+    int rc;
+    if ((rc = hip_check(g, hipSetDevice(device), "hipSetDevice")) ||
+        (rc = hip_check(g, hipStreamCreate(&g->own_stream), "hipStreamCreate")))
+        return rc;
+    g->stream = g->own_stream;
+    if ((rc = hip_check(g, (hipError_t)sweep_lds_attributes(g->max_lds_per_block), "LDS attributes")))
+        return rc;
+    g->doms.resize(ndom);
+    for (int i = 0; i < ndom; ++i) {
+        Domain &d = g->doms[i];
+        d.nx = nx[i];
+        d.ny = ny[i];
+        d.nz = nz[i];
+        d.nsub = nsub[i];
+        d.dx = dx[i];
+        d.nvox = (size_t)nx[i] * ny[i] * nz[i];
+        d.dval.assign(d.nsub, 100.0);  // resize_space's default (BioFVM_parallel.cpp:56)
+        d.dact.assign(d.nsub, 1);      // resize_densities' default (BioFVM_microenvironment.cpp:743)
+        const size_t bytes = d.nvox * d.nsub * g->elem;
+        if ((rc = hip_check(g, hipMalloc(&d.field[0], bytes), "field buffer")) ||
+            (rc = hip_check(g, hipMalloc(&d.field[1], bytes), "field buffer")) ||
+            (rc = hip_check(g, hipMalloc(&d.d_dval, d.nsub * g->elem), "dirichlet values")) ||
+            (rc = hip_check(g, hipMalloc((void **)&d.d_dact, d.nsub), "dirichlet flags")) ||
+            (rc = hip_check(g, hipMemsetAsync(d.field[0], 0, bytes, g->stream), "zero field")) ||
+            (rc = upload_dirichlet(g, d)))
+            return rc;
+    }
+    g->desc_dirty = true;
+    return BFVMB_OK;
+}
+
+void group_free(Group *g)
+{
+    (void)hipSetDevice(g->device);
+    if (g->stream)
+        (void)hipStreamSynchronize(g->stream);
+    clear_graphs(g);
+    for (Domain &d : g->doms) {
+        free_agents(d);
+        (void)hipFree(d.field[0]);
+        (void)hipFree(d.field[1]);
+        (void)hipFree(d.coef);
+        (void)hipFree(d.d_dval);
+        (void)hipFree(d.d_dact);
+    }
+    g->doms.clear();
+    (void)hipFree(g->d_desc);
+    (void)hipFree(g->d_prefix);
+    (void)hipFree(g->stage);
+    if (g->own_stream)
+        (void)hipStreamDestroy(g->own_stream);
+    g->d_desc = nullptr;
+    g->d_prefix = nullptr;
+    g->stage = nullptr;
+    g->own_stream = g->stream = nullptr;
+}
+
+int group_set_stream(Group *g, void *stream)
+{
+    int rc = hip_check(g, hipStreamSynchronize(g->stream), "stream switch");
+    g->stream = stream ? static_cast<hipStream_t>(stream) : g->own_stream;
+    return rc;
+}
+
+static bool bad_index(Group *g, int i)
+{
+    return i < 0 || i >= (int)g->doms.size();
+}
+
+int domain_set_coefficients(Group *g, int i, const double *D, const double *decay, double dt)
+{
+    if (bad_index(g, i) || !D || !decay || !(dt > 0.0))
+        return set_error(g, BFVMB_ERR_ARG, "set_coefficients: bad argument");
+    Domain &d = g->doms[i];
+    const int ns = d.nsub;
+    const int n[3] = {d.nx, d.ny, d.nz};
+    std::vector<double> c1(ns), c2(ns), c3(ns), c3a(ns);
+    lod_constants(ns, D, decay, dt, d.dx, c1.data(), c2.data(), c3.data(), c3a.data());
+    const CoefLayout L = coef_layout(d);
+    std::vector<double> blob(L.total);
+    for (int s = 0; s < ns; ++s)
+        blob[L.c1 + s] = c1[s];
+    for (int a = 0; a < 3; ++a) {
+        std::vector<double> den((size_t)ns * n[a]), cc((size_t)ns * n[a]);
+        thomas_axis(n[a], ns, c1.data(), c2.data(), c3.data(), c3a.data(), den.data(), cc.data());
+        for (size_t k = 0; k < den.size(); ++k) {
+            blob[L.den[a] + k] = den[k];
+            blob[L.cc[a] + k] = cc[k];
+            blob[L.rden[a] + k] = 1.0 / den[k];
+        }
+    }
+    const size_t bytes = blob.size() * sizeof(double);
+    int rc;
+    if (!d.coef) {
+        if ((rc = hip_check(g, hipMalloc((void **)&d.coef, bytes), "coefficient block")))
+            return rc;
+        g->desc_dirty = true;
+    }
+    if ((rc = hip_check(g, hipMemcpyAsync(d.coef, blob.data(), bytes, hipMemcpyHostToDevice, g->stream), "coefficients")) ||
+        (rc = hip_check(g, hipStreamSynchronize(g->stream), "coefficient sync")))
+        return rc;
+    d.D.assign(D, D + ns);
+    d.decay.assign(decay, decay + ns);
+    d.dt = dt;
+    d.have_coef = true;
+    return BFVMB_OK;
+}
+
+int domain_set_dirichlet(Group *g, int i, const double *value, const uint8_t *active)
+{
+    if (bad_index(g, i) || !value || !active)
+        return set_error(g, BFVMB_ERR_ARG, "set_dirichlet: bad argument");
+    Domain &d = g->doms[i];
+    d.dval.assign(value, value + d.nsub);
+    for (int s = 0; s < d.nsub; ++s)
+        d.dact[s] = active[s] ? 1 : 0;
+    return upload_dirichlet(g, d);
+}
+
+static int ensure_stage(Group *g, size_t bytes)
+{
+    if (g->stage && g->stage_bytes >= bytes)
+        return BFVMB_OK;
+    (void)hipFree(g->stage);
+    g->stage = nullptr;
+    g->stage_bytes = 0;
+    int rc = hip_check(g, hipMalloc(&g->stage, bytes), "staging buffer");
+    if (!rc)
+        g->stage_bytes = bytes;
+    return rc;
+}
+
+int domain_upload(Group *g, int i, const double *rho)
+{
+    if (bad_index(g, i) || !rho)
+        return set_error(g, BFVMB_ERR_ARG, "upload_density: bad argument");
+    Domain &d = g->doms[i];
+    const size_t bytes = d.nvox * d.nsub * sizeof(double);
+    int rc;
+    if ((rc = ensure_stage(g, bytes)) ||
+        (rc = hip_check(g, hipMemcpyAsync(g->stage, rho, bytes, hipMemcpyHostToDevice, g->stream), "density upload")) ||
+        (rc = launch_aos_to_soa(g, d, static_cast<const double *>(g->stage), d.field[g->cur])))
+        return rc;
+    return hip_check(g, hipStreamSynchronize(g->stream), "upload sync");
+}
+
+int domain_download(Group *g, int i, double *rho)
+{
+    if (bad_index(g, i) || !rho)
+        return set_error(g, BFVMB_ERR_ARG, "download_density: bad argument");
+    Domain &d = g->doms[i];
+    const size_t bytes = d.nvox * d.nsub * sizeof(double);
+    int rc;
+    if ((rc = ensure_stage(g, bytes)) ||
+        (rc = launch_soa_to_aos(g, d, d.field[g->cur], static_cast<double *>(g->stage))) ||
+        (rc = hip_check(g, hipMemcpyAsync(rho, g->stage, bytes, hipMemcpyDeviceToHost, g->stream), "density download")))
+        return rc;
+    return hip_check(g, hipStreamSynchronize(g->stream), "download sync");
+}
+
+int domain_set_agents(Group *g, int i, int n, const int *voxel, const double *temp1, const double *temp2,
+                      const uint8_t *active)
+{
+    if (bad_index(g, i) || n < 0 || (n > 0 && (!voxel || !temp1 || !temp2)))
+        return set_error(g, BFVMB_ERR_ARG, "set_agents: bad argument");
+    Domain &d = g->doms[i];
+    const int ns = d.nsub;
+    std::vector<int> order;
+    order.reserve(n);
+    for (int a = 0; a < n; ++a) {
+        if (active && !active[a])
+            continue;
+        if (voxel[a] < 0 || (size_t)voxel[a] >= d.nvox)
+            return set_error(g, BFVMB_ERR_ARG, "set_agents: voxel index out of range");
+        order.push_back(a);
+    }
+    // list order is kept inside a voxel, as the reference's loop applies them
+    std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return voxel[p] < voxel[q]; });
+    const int nact = (int)order.size();
+    std::vector<int> seg_start;
+    std::vector<long long> seg_off;
+    std::vector<double> t1((size_t)ns * nact), t2((size_t)ns * nact);
+    const long long nyz = (long long)d.ny * d.nz, nxy = (long long)d.nx * d.ny;
+    for (int k = 0; k < nact; ++k) {
+        const int a = order[k];
+        if (k == 0 || voxel[a] != voxel[order[k - 1]]) {
+            const long long v = voxel[a];
+            const long long x = v / nyz, rem = v - x * nyz, y = rem / d.nz, z = rem - y * d.nz;
+            seg_start.push_back(k);
+            seg_off.push_back(z * nxy + y * d.nx + x);
+        }
+        for (int s = 0; s < ns; ++s) {
+            t1[(size_t)s * nact + k] = temp1[(size_t)a * ns + s];
+            t2[(size_t)s * nact + k] = temp2[(size_t)a * ns + s];
+        }
+    }
+    seg_start.push_back(nact);
+    int rc = hip_check(g, hipStreamSynchronize(g->stream), "agents sync");
+    if (rc)
+        return rc;
+    free_agents(d);
+    g->desc_dirty = true;
+    if (nact == 0)
+        return BFVMB_OK;
+    d.nseg = (int)seg_off.size();
+    d.nact = nact;
+    if ((rc = hip_check(g, hipMalloc((void **)&d.seg_start, seg_start.size() * sizeof(int)), "agent segments")) ||
+        (rc = hip_check(g, hipMalloc((void **)&d.seg_off, seg_off.size() * sizeof(long long)), "agent voxels")) ||
+        (rc = hip_check(g, hipMalloc((void **)&d.t1, t1.size() * sizeof(double)), "agent temp1")) ||
+        (rc = hip_check(g, hipMalloc((void **)&d.t2, t2.size() * sizeof(double)), "agent temp2")) ||
+        (rc = hip_check(g, hipMemcpy(d.seg_start, seg_start.data(), seg_start.size() * sizeof(int), hipMemcpyHostToDevice), "agent upload")) ||
+        (rc = hip_check(g, hipMemcpy(d.seg_off, seg_off.data(), seg_off.size() * sizeof(long long), hipMemcpyHostToDevice), "agent upload")) ||
+        (rc = hip_check(g, hipMemcpy(d.t1, t1.data(), t1.size() * sizeof(double), hipMemcpyHostToDevice), "agent upload")) ||
+        (rc = hip_check(g, hipMemcpy(d.t2, t2.data(), t2.size() * sizeof(double), hipMemcpyHostToDevice), "agent upload"))) {
+        free_agents(d);
+        return rc;
+    }
+    return BFVMB_OK;
+}
+
+static int eager_steps(Group *g, int nsteps, bool with_agents)
+{
+    for (int s = 0; s < nsteps; ++s) {
+        int rc;
+        if (with_agents && (rc = launch_agents(g)))
+            return rc;
+        for (int axis = 0; axis < 3; ++axis)
+            if ((rc = launch_sweep(g, axis)))
+                return rc;
+    }
+    return BFVMB_OK;
+}
+
+int group_steps(Group *g, int nsteps, bool with_agents, bool use_graph)
+{
+    if (nsteps < 0)
+        return set_error(g, BFVMB_ERR_ARG, "nsteps < 0");
+    for (const Domain &d : g->doms)
+        if (!d.have_coef)
+            return set_error(g, BFVMB_ERR_STATE, "set_coefficients has not been called for every domain");
+    int rc = sync_descriptors(g);
+    if (rc || nsteps == 0)
+        return rc;
+    if (!use_graph || g->stream == nullptr)
+        return eager_steps(g, nsteps, with_agents);
+    // the whole run as one graph: nsteps x (agents, x, y, z) kernel nodes, replayed as one launch
+    const int start = g->cur;
+    const auto key = std::make_tuple(nsteps, start, with_agents ? 1 : 0);
+    auto it = g->graphs.find(key);
+    if (it == g->graphs.end()) {
+        hipGraph_t graph = nullptr;
+        if ((rc = hip_check(g, hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal), "graph capture")))
+            return rc;
+        const int rc_steps = eager_steps(g, nsteps, with_agents);
+        const hipError_t e = hipStreamEndCapture(g->stream, &graph);
+        g->cur = start;
+        if (rc_steps)
+            return rc_steps;
+        if ((rc = hip_check(g, e, "graph capture end")))
+            return rc;
+        hipGraphExec_t exec = nullptr;
+        rc = hip_check(g, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
+        (void)hipGraphDestroy(graph);
+        if (rc)
+            return rc;
+        it = g->graphs.emplace(key, exec).first;
+    }
+    if ((rc = hip_check(g, hipGraphLaunch(it->second, g->stream), "graph launch")))
+        return rc;
+    g->cur = start ^ (nsteps & 1);  // three buffer swaps per step
+    return BFVMB_OK;
+}
+
+int group_launch_phase(Group *g, int phase, int count)
+{
+    if (phase < 0 || phase > 3 || count < 0)
+        return set_error(g, BFVMB_ERR_ARG, "launch_phase: bad argument");
+    if (phase > 0)
+        for (const Domain &d : g->doms)
+            if (!d.have_coef)
+                return set_error(g, BFVMB_ERR_STATE, "set_coefficients has not been called for every domain");
+    for (int i = 0; i < count; ++i) {
+        int rc = phase == 0 ? launch_agents(g) : launch_sweep(g, phase - 1);
+        if (rc)
+            return rc;
+    }
+    return BFVMB_OK;
+}
+
+int group_set_launch(Group *g, int sweep, long long grid, int bpc, int ring)
+{
+    if (sweep < -1 || sweep > 2 || grid < 0 || bpc < 0 || bpc > 8 || ring < 0 || ring > 64)
+        return set_error(g, BFVMB_ERR_ARG, "set_launch: bad argument");
+    for (int a = 0; a < 3; ++a)
+        if (sweep < 0 || sweep == a) {
+            g->launch[a].grid = grid;
+            g->launch[a].lds_blocks_per_cu = bpc;
+            g->launch[a].ring = ring;
+        }
+    clear_graphs(g);
+    return BFVMB_OK;
+}
+
+// Autotuner: for each sweep, time the candidate launch shapes (LDS wavefronts per CU; ring
+// depth for y/z) over `reps` launches with HIP events and keep the fastest.  The fields are
+// saved before and restored after, so tuning does not advance the simulation.
+int group_autotune(Group *g, int reps, int *chosen)
+{
+    if (reps < 1)
+        reps = 3;
+    for (const Domain &d : g->doms)
+        if (!d.have_coef)
+            return set_error(g, BFVMB_ERR_STATE, "set_coefficients has not been called for every domain");
+    int rc = sync_descriptors(g);
+    if (rc)
+        return rc;
+    const int start = g->cur;
+    std::vector<void *> saved(g->doms.size(), nullptr);
+    for (size_t i = 0; i < g->doms.size(); ++i) {
+        const Domain &d = g->doms[i];
+        const size_t bytes = d.nvox * d.nsub * g->elem;
+        if ((rc = hip_check(g, hipMalloc(&saved[i], bytes), "autotune save")) ||
+            (rc = hip_check(g, hipMemcpyAsync(saved[i], d.field[start], bytes, hipMemcpyDeviceToDevice, g->stream),
+                            "autotune save")))
+            break;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!rc)
+        rc = hip_check(g, hipEventCreate(&e0), "event");
+    if (!rc)
+        rc = hip_check(g, hipEventCreate(&e1), "event");
+    for (int axis = 0; axis < 3 && !rc; ++axis) {
+        float best = 1e30f;
+        LaunchShape best_shape;
+        const int rings[3] = {16, 32, 64};
+        for (int bpc = 1; bpc <= 3 && !rc; ++bpc)
+            for (int r = 0; r < (axis == 0 ? 1 : 3) && !rc; ++r) {
+                g->launch[axis] = LaunchShape();
+                g->launch[axis].lds_blocks_per_cu = bpc;
+                g->launch[axis].ring = axis == 0 ? 0 : rings[r];
+                if ((rc = launch_sweep(g, axis)))  // warm-up
+                    break;
+                (void)hipEventRecord(e0, g->stream);
+                for (int k = 0; k < reps && !rc; ++k)
+                    rc = launch_sweep(g, axis);
+                (void)hipEventRecord(e1, g->stream);
+                if (!rc)
+                    rc = hip_check(g, hipEventSynchronize(e1), "autotune timing");
+                float ms = 0.f;
+                if (!rc && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best) {
+                    best = ms;
+                    best_shape = g->launch[axis];
+                }
+            }
+        g->launch[axis] = best_shape;
+        if (chosen) {
+            chosen[3 * axis] = best_shape.lds_blocks_per_cu;
+            chosen[3 * axis + 1] = best_shape.ring;
+            chosen[3 * axis + 2] = (int)(best * 1000.f / reps);  // microseconds per launch
+        }
+    }
+    if (e0)
+        (void)hipEventDestroy(e0);
+    if (e1)
+        (void)hipEventDestroy(e1);
+    for (size_t i = 0; i < g->doms.size(); ++i) {
+        if (!saved[i])
+            continue;
+        const Domain &d = g->doms[i];
+        if (!rc)
+            rc = hip_check(g, hipMemcpyAsync(d.field[start], saved[i], d.nvox * d.nsub * g->elem,
+                                             hipMemcpyDeviceToDevice, g->stream), "autotune restore");
+    }
+    int rc2 = hip_check(g, hipStreamSynchronize(g->stream), "autotune sync");
+    for (void *p : saved)
+        (void)hipFree(p);
+    g->cur = start;
+    clear_graphs(g);
+    return rc ? rc : rc2;
+}
+
+}  // namespace bfvmb

@@ -37,8 +37,6 @@
 namespace bfvmb {
 namespace {
 
-constexpr int kWave = 64;
-
 // Thomas tables are read through the constant address space: the index is uniform, so
 // every read becomes a scalar load that never waits on the vector-memory counter that
 // tracks the in-flight field rows.
@@ -87,55 +85,105 @@ __host__ __device__ __forceinline__ int park_boundary(int n, int L, int P)
     return lo < n ? lo : n;
 }
 
+// Domain of global tile t: the largest i with prefix[i] <= t (prefix has ndom+1 entries).
+__device__ __forceinline__ int find_domain(const long long *prefix, int ndom, long long t)
+{
+    int lo = 0, hi = ndom - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Elements per line kept in the y/z line store for a line of n with `budget` LDS bytes
+// (after the {den, rden}, cc tables of 24 bytes per element).
+template <class T>
+__host__ __device__ __forceinline__ int yz_line_store(int n, long long budget)
+{
+    long long L = (budget - 24LL * n) / (kWave * (long long)sizeof(T));
+    if (L > n)
+        L = n;
+    return L < 0 ? 0 : (int)L;
+}
+
+// Columns per row kept in the x line store (a multiple of the 16-column chunk, odd pitch),
+// after the 64 x 17 staging tile.
+template <class T>
+__host__ __device__ __forceinline__ int x_line_store(int nx, long long budget)
+{
+    const long long stg = (long long)kWave * 17 * sizeof(T);
+    long long L = budget > stg ? (budget - stg) / (kWave * (long long)sizeof(T)) - 1 : 0;
+    L = L / 16 * 16;
+    const long long need = (nx + 15) / 16 * 16;
+    if (L > need)
+        L = need;
+    return L < 0 ? 0 : (int)L;
+}
+
 // ------------------------------------------------------------------------------ y / z
 // Block = one wavefront = 64 lines of one substrate.  LDS: {den, rden}[n], cc[n] of the
-// block's substrate, then the line store [L][64].
+// block's substrate, then the line store [L][64].  P = rows in flight per lane (ring
+// depth); table entries are read in batches of TB per wait.
+constexpr int TB = 16;
+
 template <class T, int AXIS, int P>
-__global__ __launch_bounds__(64) void k_sweep_yz(SweepArgs<T> a, int L)
+__global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict__ descs,
+                                                 const long long *__restrict__ prefix, int ndom, int parity,
+                                                 int budget)
 {
+    static_assert(P % TB == 0, "ring depth must be a multiple of the table batch");
     extern __shared__ unsigned char smem[];
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const size_t nxy = (size_t)nx * ny;
-    const size_t nvox = nxy * nz;
-    const int n = AXIS == 1 ? ny : nz;
-    double2 *tab2 = reinterpret_cast<double2 *>(smem);
-    double *tabc = reinterpret_cast<double *>(tab2 + n);
-    T *ls = reinterpret_cast<T *>(tabc + n);
-    const long long lines_per_s = AXIS == 1 ? (long long)nx * nz : (long long)nxy;
-    const long long tiles_per_s = (lines_per_s + kWave - 1) / kWave;
-    const long long ntiles = tiles_per_s * a.nsub;
-    const size_t stride = AXIS == 1 ? (size_t)nx : nxy;
     const int lane = threadIdx.x;
-    const int lo = park_boundary(n, L, P);
-    int tab_s = -1;
+    const long long ntiles = prefix[ndom];
+    int tab_dom = -1, tab_s = -1;
 
     for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const int s = (int)(tile / tiles_per_s);
-        if (s != tab_s) {  // stage this substrate's Thomas tables
+        const int di = find_domain(prefix, ndom, tile);
+        const DomainDesc<T> &D = descs[di];
+        const int nx = D.nx, ny = D.ny, nz = D.nz;
+        const size_t nxy = (size_t)nx * ny;
+        const size_t nvox = nxy * nz;
+        const int n = AXIS == 1 ? ny : nz;
+        double2 *tab2 = reinterpret_cast<double2 *>(smem);
+        double *tabc = reinterpret_cast<double *>(tab2 + n);
+        T *ls = reinterpret_cast<T *>(tabc + n);
+        const int L = yz_line_store<T>(n, budget);
+        const long long lines_per_s = AXIS == 1 ? (long long)nx * nz : (long long)nxy;
+        const long long tiles_per_s = (lines_per_s + kWave - 1) / kWave;
+        const size_t stride = AXIS == 1 ? (size_t)nx : nxy;
+        const int lo = park_boundary(n, L, P);
+        const long long lt = tile - prefix[di];
+        const int s = (int)(lt / tiles_per_s);
+        if (di != tab_dom || s != tab_s) {  // stage this substrate's Thomas tables
             __syncthreads();
-            const double *gd = a.tab.den + (size_t)s * n, *gr = a.tab.rden + (size_t)s * n;
-            const double *gc = a.tab.cc + (size_t)s * n;
+            const double *gd = D.tab[AXIS].den + (size_t)s * n, *gr = D.tab[AXIS].rden + (size_t)s * n;
+            const double *gc = D.tab[AXIS].cc + (size_t)s * n;
             for (int i = lane; i < n; i += kWave) {
                 tab2[i] = make_double2(gd[i], gr[i]);
                 tabc[i] = gc[i];
             }
             __syncthreads();
+            tab_dom = di;
             tab_s = s;
         }
-        long long line = (tile - (long long)s * tiles_per_s) * kWave + lane;
+        long long line = (lt - (long long)s * tiles_per_s) * kWave + lane;
         if (line >= lines_per_s)
             line = lines_per_s - 1;  // idle lanes repeat the last line (same values, same stores)
         const int o = (int)(line / nx);
         const int x = (int)(line - (long long)o * nx);
         const size_t base = (size_t)s * nvox + (AXIS == 1 ? (size_t)o * nxy + x : (size_t)line);
         const int on = AXIS == 1 ? nz : ny;
-        const bool act = a.dact[s] != 0;
-        const T dv = a.dval[s];
-        const T *src = a.in + base;
-        T *dst = a.out + base;
+        const bool act = D.dact[s] != 0;
+        const T dv = D.dval[s];
+        const T *src = D.field[parity] + base;
+        T *dst = D.field[parity ^ 1] + base;
         // a line on an active face is computed along but stored as the Dirichlet value
         const bool face = act && (x == 0 || x == nx - 1 || o == 0 || o == on - 1);
-        const double c1 = a.c1[s];
+        const double c1 = D.c1[s];
 
         // ---- forward elimination; a ring of P rows is in flight (reads past the end
         // re-read row n-1 and are discarded)
@@ -146,59 +194,55 @@ __global__ __launch_bounds__(64) void k_sweep_yz(SweepArgs<T> a, int L)
         double y = -0.0;
         double amin = 1.0;
         int k0 = 0;
-        // [0, lo): parked in the output buffer
-        for (; k0 + P <= lo; k0 += P) {
-            double2 t[P];  // the chunk's table entries in one batch of LDS reads
+        // one full chunk of P elements; PARK: results to the output buffer, else to LDS
+        auto chunk = [&](auto park_tag) {
+            constexpr bool PARK = decltype(park_tag)::value;
 #pragma unroll
-            for (int j = 0; j < P; ++j)
-                t[j] = tab2[k0 + j];
+            for (int jb = 0; jb < P; jb += TB) {
+                double2 t[TB];
 #pragma unroll
-            for (int j = 0; j < P; ++j) {
-                const int k = k0 + j;
-                const double v = (double)buf[j];
-                buf[j] = src[(size_t)min(k + P, n - 1) * stride];
-                y = fwd<T>(c1, y, v, t[j].x, t[j].y, amin);
-                dst[(size_t)k * stride] = face ? dv : (T)y;
-            }
-        }
-        if (k0 < lo) {  // only when every element is parked and n is not a multiple of P
+                for (int jj = 0; jj < TB; ++jj)
+                    t[jj] = tab2[k0 + jb + jj];
 #pragma unroll
-            for (int j = 0; j < P; ++j) {
-                const int k = k0 + j;
-                if (k < lo) {
-                    const double2 t = tab2[k];
-                    y = fwd<T>(c1, y, (double)buf[j], t.x, t.y, amin);
-                    dst[(size_t)k * stride] = face ? dv : (T)y;
+                for (int jj = 0; jj < TB; ++jj) {
+                    const int j = jb + jj;
+                    const int k = k0 + j;
+                    const double v = (double)buf[j];
+                    buf[j] = src[(size_t)min(k + P, n - 1) * stride];
+                    y = fwd<T>(c1, y, v, t[jj].x, t[jj].y, amin);
+                    if constexpr (PARK)
+                        dst[(size_t)k * stride] = face ? dv : (T)y;
+                    else
+                        ls[(size_t)(k - lo) * kWave + lane] = (T)y;
                 }
             }
+        };
+        // the last, partial chunk of a phase (uniform per-element conditions)
+        auto tail = [&](int end, auto park_tag) {
+            constexpr bool PARK = decltype(park_tag)::value;
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                const int k = k0 + j;
+                if (k < end) {
+                    const double2 t = tab2[k];
+                    y = fwd<T>(c1, y, (double)buf[j], t.x, t.y, amin);
+                    if constexpr (PARK)
+                        dst[(size_t)k * stride] = face ? dv : (T)y;
+                    else
+                        ls[(size_t)(k - lo) * kWave + lane] = (T)y;
+                }
+            }
+        };
+        for (; k0 + P <= lo; k0 += P)  // [0, lo): parked in the output buffer
+            chunk(std::true_type());
+        if (k0 < lo) {  // only when every element is parked and n is not a multiple of P
+            tail(lo, std::true_type());
             k0 = lo;
         }
-        // [lo, n): LDS line store
-        for (; k0 + P <= n; k0 += P) {
-            double2 t[P];
-#pragma unroll
-            for (int j = 0; j < P; ++j)
-                t[j] = tab2[k0 + j];
-#pragma unroll
-            for (int j = 0; j < P; ++j) {
-                const int k = k0 + j;
-                const double v = (double)buf[j];
-                buf[j] = src[(size_t)min(k + P, n - 1) * stride];
-                y = fwd<T>(c1, y, v, t[j].x, t[j].y, amin);
-                ls[(size_t)(k - lo) * kWave + lane] = (T)y;
-            }
-        }
-        if (k0 < n) {
-#pragma unroll
-            for (int j = 0; j < P; ++j) {
-                const int k = k0 + j;
-                if (k < n) {
-                    const double2 t = tab2[k];
-                    y = fwd<T>(c1, y, (double)buf[j], t.x, t.y, amin);
-                    ls[(size_t)(k - lo) * kWave + lane] = (T)y;
-                }
-            }
-        }
+        for (; k0 + P <= n; k0 += P)  // [lo, n): LDS line store
+            chunk(std::false_type());
+        if (k0 < n)
+            tail(n, std::false_type());
         if constexpr (sizeof(T) == 8) {
             // a numerator below 2^-960 somewhere in the tile: redo the forward pass with
             // true divisions (same results everywhere else)
@@ -217,24 +261,24 @@ __global__ __launch_bounds__(64) void k_sweep_yz(SweepArgs<T> a, int L)
         // ---- back substitution; the last element is y[n-1] itself
         double xv = y;
         dst[(size_t)(n - 1) * stride] = act ? dv : (T)xv;
-        // LDS part, k = n-2 .. lo
-        for (int kt = n - 2; kt >= lo; kt -= P) {
-            if (kt - (P - 1) >= lo) {
-                double v[P], c[P];
+        // LDS part, k = n-2 .. lo, in batches of TB reads
+        for (int kt = n - 2; kt >= lo; kt -= TB) {
+            if (kt - (TB - 1) >= lo) {
+                double v[TB], c[TB];
 #pragma unroll
-                for (int j = 0; j < P; ++j) {
+                for (int j = 0; j < TB; ++j) {
                     v[j] = (double)ls[(size_t)(kt - j - lo) * kWave + lane];
                     c[j] = tabc[kt - j];
                 }
 #pragma unroll
-                for (int j = 0; j < P; ++j) {
+                for (int j = 0; j < TB; ++j) {
                     const int k = kt - j;
                     xv = fma(-c[j], xv, v[j]);
                     dst[(size_t)k * stride] = (face || (act && k == 0)) ? dv : (T)xv;
                 }
             } else {
 #pragma unroll
-                for (int j = 0; j < P; ++j) {
+                for (int j = 0; j < TB; ++j) {
                     const int k = kt - j;
                     if (k >= lo) {
                         xv = fma(-tabc[k], xv, (double)ls[(size_t)(k - lo) * kWave + lane]);
@@ -252,12 +296,20 @@ __global__ __launch_bounds__(64) void k_sweep_yz(SweepArgs<T> a, int L)
             int kt = top;
             for (; kt - (P - 1) >= 0; kt -= P) {
 #pragma unroll
-                for (int j = 0; j < P; ++j) {
-                    const int k = kt - j;
-                    const double v = (double)buf[j];
-                    buf[j] = dst[(size_t)max(k - P, 0) * stride];
-                    xv = fma(-tabc[k], xv, v);
-                    dst[(size_t)k * stride] = (face || (act && k == 0)) ? dv : (T)xv;
+                for (int jb = 0; jb < P; jb += TB) {
+                    double c[TB];
+#pragma unroll
+                    for (int jj = 0; jj < TB; ++jj)
+                        c[jj] = tabc[kt - jb - jj];
+#pragma unroll
+                    for (int jj = 0; jj < TB; ++jj) {
+                        const int j = jb + jj;
+                        const int k = kt - j;
+                        const double v = (double)buf[j];
+                        buf[j] = dst[(size_t)max(k - P, 0) * stride];
+                        xv = fma(-c[jj], xv, v);
+                        dst[(size_t)k * stride] = (face || (act && k == 0)) ? dv : (T)xv;
+                    }
                 }
             }
             if (kt >= 0) {
@@ -279,7 +331,9 @@ __global__ __launch_bounds__(64) void k_sweep_yz(SweepArgs<T> a, int L)
 // [64][W+1] and line store [64][L+1] (odd pitches keep the transposed accesses
 // conflict-free).  Column chunks are W = 16 wide; PC chunks of loads are kept in flight.
 template <class T, int PC>
-__global__ __launch_bounds__(64) void k_sweep_x(SweepArgs<T> a, int L)
+__global__ __launch_bounds__(64) void k_sweep_x(const DomainDesc<T> *__restrict__ descs,
+                                                const long long *__restrict__ prefix, int ndom, int parity,
+                                                int budget)
 {
     constexpr int W = 16;
     constexpr int RPI = kWave / W;   // rows per load instruction (4)
@@ -287,35 +341,38 @@ __global__ __launch_bounds__(64) void k_sweep_x(SweepArgs<T> a, int L)
     extern __shared__ unsigned char smem[];
     T *stg = reinterpret_cast<T *>(smem);  // [64][W+1]
     T *ls = stg + kWave * (W + 1);          // [64][L+1]
-    const int Lp = L + 1;
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const size_t nvox = (size_t)nx * ny * nz;
-    const int nrows = ny * nz;
-    const int tiles_per_s = (nrows + kWave - 1) / kWave;
-    const long long ntiles = (long long)tiles_per_s * a.nsub;
     const int lane = threadIdx.x;
-    const int nch = (nx + W - 1) / W;
-    int cl = nch - L / W;  // chunks [cl, nch) live in the line store; L is a multiple of W
-    if (cl < 0)
-        cl = 0;
     const int lrow = lane / W, lcol = lane % W;  // position of this lane in a load instruction
+    const long long ntiles = prefix[ndom];
 
     for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int s = (int)(t / tiles_per_s);
-        const int r0 = (int)(t - (long long)s * tiles_per_s) * kWave;
+        const int di = find_domain(prefix, ndom, t);
+        const DomainDesc<T> &D = descs[di];
+        const int nx = D.nx, ny = D.ny, nz = D.nz;
+        const size_t nvox = (size_t)nx * ny * nz;
+        const int nrows = ny * nz;
+        const int tiles_per_s = (nrows + kWave - 1) / kWave;
+        const int L = x_line_store<T>(nx, budget);
+        const int Lp = L + 1;
+        const int nch = (nx + W - 1) / W;
+        int cl = nch - L / W;  // chunks [cl, nch) live in the line store; L is a multiple of W
+        if (cl < 0)
+            cl = 0;
+        const long long lt = t - prefix[di];
+        const int s = (int)(lt / tiles_per_s);
+        const int r0 = (int)(lt - (long long)s * tiles_per_s) * kWave;
         const int rows_here = min(kWave, nrows - r0);
         const int my_row = r0 + min(lane, rows_here - 1);
-        const bool row_ok = lane < rows_here;
         const int yy = my_row % ny, zz = my_row / ny;
-        const bool act = a.dact[s] != 0;
-        const T dv = a.dval[s];
+        const bool act = D.dact[s] != 0;
+        const T dv = D.dval[s];
         const bool whole = act && (yy == 0 || yy == ny - 1 || zz == 0 || zz == nz - 1);
-        const T *src = a.in + (size_t)s * nvox + (size_t)r0 * nx;
-        T *dst = a.out + (size_t)s * nvox + (size_t)r0 * nx;
-        const double c1 = a.c1[s];
-        const ctab den = as_ctab(a.tab.den + (size_t)s * nx);
-        const ctab cc = as_ctab(a.tab.cc + (size_t)s * nx);
-        const ctab rden = as_ctab(a.tab.rden + (size_t)s * nx);
+        const T *src = D.field[parity] + (size_t)s * nvox + (size_t)r0 * nx;
+        T *dst = D.field[parity ^ 1] + (size_t)s * nvox + (size_t)r0 * nx;
+        const double c1 = D.c1[s];
+        const ctab den = as_ctab(D.tab[0].den + (size_t)s * nx);
+        const ctab cc = as_ctab(D.tab[0].cc + (size_t)s * nx);
+        const ctab rden = as_ctab(D.tab[0].rden + (size_t)s * nx);
 
         // chunk c of this tile <-> registers: instruction i covers rows RPI*i .. RPI*i+3,
         // columns c*W .. c*W+15; out-of-range lanes re-read an in-range element
@@ -453,98 +510,119 @@ __global__ __launch_bounds__(64) void k_sweep_x(SweepArgs<T> a, int L)
             store_chunk(c);
             __syncthreads();
         }
-        (void)row_ok;
     }
-}
-
-template <class T>
-SweepArgs<T> make_args(bfvmb_ctx *ctx, int axis)
-{
-    CoefLayout L = coef_layout(ctx);
-    const double *coef = ctx->coef;
-    SweepArgs<T> a;
-    a.in = static_cast<const T *>(ctx->field[ctx->cur]);
-    a.out = static_cast<T *>(ctx->field[ctx->cur ^ 1]);
-    a.c1 = coef + L.c1;
-    a.tab.den = coef + L.den[axis];
-    a.tab.cc = coef + L.cc[axis];
-    a.tab.rden = coef + L.rden[axis];
-    a.dval = static_cast<const T *>(ctx->d_dval);
-    a.dact = ctx->d_dact;
-    a.nx = ctx->nx;
-    a.ny = ctx->ny;
-    a.nz = ctx->nz;
-    a.nsub = ctx->nsub;
-    return a;
 }
 
 constexpr size_t kLdsBytes = 160 * 1024;
 
-template <class T>
-int launch_sweep_t(bfvmb_ctx *ctx, int axis)
+struct Shape {
+    long long grid;  // 0 = one block per tile
+    int bpc;         // sweep wavefronts sharing one CU's LDS
+    int ring;        // y/z: rows in flight per lane
+};
+
+Shape shape_of(const Group *g, int axis)
 {
-    SweepArgs<T> a = make_args<T>(ctx, axis);
-    const int bpc = ctx->lds_blocks_per_cu > 0 ? ctx->lds_blocks_per_cu : 1;
-    const size_t budget = std::min(kLdsBytes / bpc, ctx->max_lds_per_block);
-    if (axis == 0) {
-        constexpr int W = 16;
-        const size_t stg = (size_t)kWave * (W + 1) * sizeof(T);
-        // line store elements per row, a multiple of W, with an odd pitch (L+1)
-        long long L = budget > stg ? (long long)((budget - stg) / (kWave * sizeof(T))) - 1 : 0;
-        L = L / W * W;
-        const long long need = ((ctx->nx + W - 1) / W) * W;
-        if (L > need)
-            L = need;
-        if (L < 0)
-            L = 0;
-        const size_t lds = stg + (size_t)kWave * (L + 1) * sizeof(T);
-        long long tiles = (long long)((ctx->ny * ctx->nz + kWave - 1) / kWave) * ctx->nsub;
-        long long grid = ctx->x_blocks > 0 ? std::min<long long>(ctx->x_blocks, tiles) : tiles;
-        hipLaunchKernelGGL((k_sweep_x<T, 3>), dim3((unsigned)grid), dim3(kWave), lds, ctx->stream, a, (int)L);
-    } else {
-        const int n = axis == 1 ? ctx->ny : ctx->nz;
-        const size_t tables = (size_t)n * 3 * sizeof(double);
-        if (tables + 16 > budget)
-            return set_error(ctx, BFVMB_ERR_ARG, "sweep: axis too long for the LDS Thomas tables");
-        long long L = (long long)((budget - tables) / (kWave * sizeof(T)));
-        if (L > n)
-            L = n;
-        const size_t lds = tables + (size_t)kWave * L * sizeof(T);
-        long long lines = axis == 1 ? (long long)ctx->nx * ctx->nz : (long long)ctx->nx * ctx->ny;
-        long long tiles = ((lines + kWave - 1) / kWave) * ctx->nsub;
-        long long grid = ctx->yz_blocks > 0 ? std::min<long long>(ctx->yz_blocks, tiles) : tiles;
-        if (axis == 1)
-            hipLaunchKernelGGL((k_sweep_yz<T, 1, 16>), dim3((unsigned)grid), dim3(kWave), lds, ctx->stream, a, (int)L);
-        else
-            hipLaunchKernelGGL((k_sweep_yz<T, 2, 16>), dim3((unsigned)grid), dim3(kWave), lds, ctx->stream, a, (int)L);
+    const LaunchShape &u = g->launch[axis];
+    Shape sh;
+    sh.grid = u.grid;
+    // defaults measured on MI355X at 400^3: y/z keep each line in one wavefront's LDS with
+    // a 64-deep ring; the x sweep shares the CU between two wavefronts
+    sh.bpc = u.lds_blocks_per_cu > 0 ? u.lds_blocks_per_cu : (axis == 0 ? 2 : 1);
+    sh.ring = u.ring > 0 ? u.ring : 64;
+    return sh;
+}
+
+template <class T, int AXIS>
+void launch_yz(const Group *g, const DomainDesc<T> *d, const long long *pre, int ndom, unsigned grid, size_t lds,
+               int ring)
+{
+    if (ring <= 16)
+        hipLaunchKernelGGL((k_sweep_yz<T, AXIS, 16>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+    else if (ring <= 32)
+        hipLaunchKernelGGL((k_sweep_yz<T, AXIS, 32>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+    else
+        hipLaunchKernelGGL((k_sweep_yz<T, AXIS, 64>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+}
+
+template <class T>
+int launch_sweep_t(Group *g, int axis)
+{
+    int rc = sync_descriptors(g);
+    if (rc)
+        return rc;
+    const int ndom = (int)g->doms.size();
+    const Shape sh = shape_of(g, axis);
+    const size_t budget = std::min(kLdsBytes / sh.bpc, g->max_lds_per_block);
+    // LDS actually needed by the largest domain, capped by the budget
+    size_t need = 0;
+    for (const Domain &d : g->doms) {
+        if (axis == 0) {
+            need = std::max(need, (size_t)kWave * 17 * sizeof(T) + (size_t)kWave * ((d.nx + 15) / 16 * 16 + 1) * sizeof(T));
+        } else {
+            const int n = axis == 1 ? d.ny : d.nz;
+            if ((size_t)n * 24 + 16 > budget)
+                return set_error(g, BFVMB_ERR_ARG, "sweep: axis too long for the LDS Thomas tables");
+            need = std::max(need, (size_t)n * 24 + (size_t)kWave * n * sizeof(T));
+        }
     }
+    const size_t lds = std::min(budget, need);
+    const long long *pre = g->d_prefix + (size_t)(axis + 1) * (ndom + 1);
+    const long long ntiles = g->prefix[axis + 1][ndom];
+    if (ntiles == 0)
+        return BFVMB_OK;
+    long long grid = sh.grid > 0 ? std::min<long long>(sh.grid, ntiles) : ntiles;
+    grid = std::min<long long>(grid, 1LL << 30);
+    const DomainDesc<T> *d = static_cast<const DomainDesc<T> *>(g->d_desc);
+    if (axis == 0)
+        hipLaunchKernelGGL((k_sweep_x<T, 3>), dim3((unsigned)grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur,
+                           (int)lds);
+    else if (axis == 1)
+        launch_yz<T, 1>(g, d, pre, ndom, (unsigned)grid, lds, sh.ring);
+    else
+        launch_yz<T, 2>(g, d, pre, ndom, (unsigned)grid, lds, sh.ring);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
-        return hip_check(ctx, e, "sweep launch");
-    ctx->cur ^= 1;
+        return hip_check(g, e, "sweep launch");
+    g->cur ^= 1;
     return BFVMB_OK;
+}
+
+template <class T, int AXIS>
+hipError_t yz_attributes(int bytes)
+{
+    hipError_t e = hipFuncSetAttribute((const void *)k_sweep_yz<T, AXIS, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)k_sweep_yz<T, AXIS, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)k_sweep_yz<T, AXIS, 64>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    return e;
 }
 
 }  // namespace
 
-int launch_sweep(bfvmb_ctx *ctx, int axis)
+int launch_sweep(Group *g, int axis)
 {
-    if (ctx->precision == BFVMB_FP64)
-        return launch_sweep_t<double>(ctx, axis);
-    return launch_sweep_t<float>(ctx, axis);
+    if (g->precision == BFVMB_FP64)
+        return launch_sweep_t<double>(g, axis);
+    return launch_sweep_t<float>(g, axis);
 }
 
 int sweep_lds_attributes(size_t lds_bytes)
 {
     // allow the device's full LDS for every sweep instantiation
     const int bytes = (int)lds_bytes;
-    hipError_t e = hipSuccess;
-    e = e ? e : hipFuncSetAttribute((const void *)k_sweep_yz<double, 1, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    e = e ? e : hipFuncSetAttribute((const void *)k_sweep_yz<double, 2, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    e = e ? e : hipFuncSetAttribute((const void *)k_sweep_yz<float, 1, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    e = e ? e : hipFuncSetAttribute((const void *)k_sweep_yz<float, 2, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    e = e ? e : hipFuncSetAttribute((const void *)k_sweep_x<double, 3>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    e = e ? e : hipFuncSetAttribute((const void *)k_sweep_x<float, 3>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    hipError_t e = yz_attributes<double, 1>(bytes);
+    if (e == hipSuccess)
+        e = yz_attributes<double, 2>(bytes);
+    if (e == hipSuccess)
+        e = yz_attributes<float, 1>(bytes);
+    if (e == hipSuccess)
+        e = yz_attributes<float, 2>(bytes);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)k_sweep_x<double, 3>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)k_sweep_x<float, 3>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     return e == hipSuccess ? 0 : (int)e;
 }
 

@@ -98,6 +98,11 @@ int bfvmb_simulate_diffusion_decay(bfvmb_ctx *ctx, int nsteps);
  * sources/sinks then one LOD step. */
 int bfvmb_run(bfvmb_ctx *ctx, int nsteps);
 
+/* bfvmb_run captured once as a HIP graph (nsteps x 4 kernel nodes) and replayed as one
+ * launch; the graph is cached per (nsteps, buffer parity) until agents, coefficients
+ * pointers or launch shapes change.  Same results as bfvmb_run. */
+int bfvmb_run_graph(bfvmb_ctx *ctx, int nsteps);
+
 /* Block the host until the context's stream is idle. */
 int bfvmb_sync(bfvmb_ctx *ctx);
 
@@ -105,17 +110,50 @@ int bfvmb_sync(bfvmb_ctx *ctx);
  * sweep; enqueued `count` times on the context's stream. */
 int bfvmb_launch_phase(bfvmb_ctx *ctx, int phase, int count);
 
-/* Launch shape of the sweep kernels (0 = library default): grid size of the y/z and of the
- * x sweep (a smaller grid strides over the line tiles), and how many sweep wavefronts
- * share one CU's LDS line store (1 = the whole LDS per wavefront).  Used by the autotuner;
- * results do not depend on it. */
-int bfvmb_set_launch(bfvmb_ctx *ctx, int yz_blocks, int x_blocks, int lds_blocks_per_cu);
+/* Launch shape of sweep `sweep` (0 = x, 1 = y, 2 = z, -1 = all three); 0 keeps the
+ * library default: grid size (a smaller grid strides over the line tiles), how many sweep
+ * wavefronts share one CU's LDS line store, and (y/z) rows in flight per lane, 16/32/64.
+ * Used by the autotuner; results do not depend on it. */
+int bfvmb_set_launch(bfvmb_ctx *ctx, int sweep, int grid_blocks, int lds_blocks_per_cu, int ring_depth);
+
+/* Autotuner: times the candidate launch shapes of each sweep (LDS wavefronts per CU 1..3;
+ * ring depth 16/32/64 for y/z) over `reps` launches with HIP events and keeps the fastest.
+ * The field is saved and restored, so tuning does not advance the simulation.  `chosen`
+ * (may be NULL) receives 3 x {wavefronts per CU, ring, microseconds per launch}. */
+int bfvmb_autotune(bfvmb_ctx *ctx, int reps, int *chosen);
 
 /* LDS bytes per workgroup and CU count of HIP device `device`. */
 int bfvmb_device_info(int device, int *lds_per_block, int *cu_count);
 
 /* Device address of the current field, [nsub][nz][ny][nx] of 8- or 4-byte reals. */
 int bfvmb_density_device_ptr(bfvmb_ctx *ctx, void **ptr);
+
+/* ---- Batches: many independent microenvironments of uneven sizes and substrate counts,
+ * advanced together.  Each phase of a step (sources/sinks, x, y, z sweep) is one kernel
+ * launch over all members, and bfvmb_batch_run can capture the whole run as one HIP graph.
+ * Member i has its own coefficients, Dirichlet, field and agents; the per-member calls
+ * mean what the single-context calls above mean.  No counterpart in the reference, whose
+ * drivers advance one Microenvironment per process (examples/test_VS.cpp:284-291). */
+typedef struct bfvmb_batch bfvmb_batch;
+int bfvmb_batch_create(bfvmb_batch **out, int n, const int *nx, const int *ny, const int *nz,
+                       const int *nsub, const double *dx, int precision, int device);
+int bfvmb_batch_destroy(bfvmb_batch *b);
+const char *bfvmb_batch_last_error(const bfvmb_batch *b);
+int bfvmb_batch_set_stream(bfvmb_batch *b, void *stream);
+int bfvmb_batch_set_coefficients(bfvmb_batch *b, int i, const double *diffusion, const double *decay,
+                                 double dt);
+int bfvmb_batch_set_dirichlet(bfvmb_batch *b, int i, const double *value, const uint8_t *active);
+int bfvmb_batch_upload_density(bfvmb_batch *b, int i, const double *rho_voxel_major);
+int bfvmb_batch_download_density(bfvmb_batch *b, int i, double *rho_voxel_major);
+int bfvmb_batch_set_agents(bfvmb_batch *b, int i, int n, const int *voxel, const double *temp1,
+                           const double *temp2, const uint8_t *active);
+/* nsteps of (sources/sinks, LOD step) for every member; use_graph != 0 replays a captured
+ * graph of the whole run. */
+int bfvmb_batch_run(bfvmb_batch *b, int nsteps, int use_graph);
+int bfvmb_batch_sync(bfvmb_batch *b);
+int bfvmb_batch_set_launch(bfvmb_batch *b, int sweep, int grid_blocks, int lds_blocks_per_cu,
+                           int ring_depth);
+int bfvmb_batch_autotune(bfvmb_batch *b, int reps, int *chosen);
 
 #ifdef __cplusplus
 }

@@ -123,16 +123,17 @@ def test_set_agents_with_temps_and_inactive():
 
 
 @pytest.mark.parametrize("name", ["chunks_130x70x33x2", "long_x_700x5x6x2", "long_z_5x6x700x2"])
-@pytest.mark.parametrize("launch", [(7, 5, 1), (0, 0, 2), (3, 11, 8)])
+@pytest.mark.parametrize("launch", [(7, 1, 16), (0, 2, 32), (3, 8, 64), (0, 4, 16)])
 def test_launch_shapes_do_not_change_results(name, launch):
-    """Strided grids and smaller LDS line stores (more of each line parked in the output
-    buffer) give the same bits as the default launch."""
+    """Strided grids, ring depths and smaller LDS line stores (more of each line parked in
+    the output buffer) give the same bits as the default launch."""
     case = _case(name)
     print(f"[gpu] launch {launch} {name}", flush=True)
     ref = orc.run(case)
     for prec in ("fp64", "fp32"):
         s = _solver(case, prec)
-        s.set_launch(*launch)
+        grid, bpc, ring = launch
+        s.set_launch(-1, grid, bpc, ring)
         s.run(case.nsteps)
         got = s.download()
         s.close()
