@@ -45,6 +45,8 @@ SIGNATURES = {
     "bfvmb_set_launch": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_int, _c.c_int]),
     "bfvmb_run_graph": (_c.c_int, [_vp, _c.c_int]),
     "bfvmb_autotune": (_c.c_int, [_vp, _c.c_int, _ip]),
+    "bfvmb_adjoint_forward": (_c.c_int, [_vp, _c.c_int, _c.c_int]),
+    "bfvmb_adjoint_backward": (_c.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "bfvmb_device_info": (_c.c_int, [_c.c_int, _ip, _ip]),
     "bfvmb_density_device_ptr": (_c.c_int, [_vp, _c.POINTER(_vp)]),
     "bfvmb_batch_create": (_c.c_int, [_c.POINTER(_vp), _c.c_int, _ip, _ip, _ip, _ip, _dp, _c.c_int, _c.c_int]),

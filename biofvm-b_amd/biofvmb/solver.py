@@ -100,6 +100,7 @@ class LODSolver:
         t1 = np.ascontiguousarray(temp1, dtype=np.float64).reshape(n, self.nsub)
         t2 = np.ascontiguousarray(temp2, dtype=np.float64).reshape(n, self.nsub)
         a = None if active is None else np.ascontiguousarray(active, dtype=np.uint8)
+        self._nagents = n
         self._check(self._lib.bfvmb_set_agents(self._h, n, L.iptr(v), L.dptr(t1), L.dptr(t2),
                                                None if a is None else L.bptr(a)), "set_agents")
 
@@ -109,6 +110,7 @@ class LODSolver:
         vol = np.ascontiguousarray(np.broadcast_to(volume, (n,)), dtype=np.float64)
         S, U, T = (np.ascontiguousarray(x, dtype=np.float64).reshape(n, self.nsub) for x in (secretion, uptake, saturation))
         a = None if active is None else np.ascontiguousarray(active, dtype=np.uint8)
+        self._nagents = n
         self._check(self._lib.bfvmb_set_agent_rates(self._h, n, L.iptr(v), L.dptr(vol), L.dptr(S), L.dptr(U), L.dptr(T),
                                                     None if a is None else L.bptr(a), float(dt)), "set_agent_rates")
 
@@ -130,6 +132,23 @@ class LODSolver:
     def run(self, nsteps: int, graph: bool = False) -> None:
         fn = self._lib.bfvmb_run_graph if graph else self._lib.bfvmb_run
         self._check(fn(self._h, nsteps), "run")
+
+    def adjoint_forward(self, nsteps: int, checkpoint_every: int = 10) -> None:
+        """``nsteps`` of ``run`` recording checkpoints for ``adjoint_backward``."""
+        self._check(self._lib.bfvmb_adjoint_forward(self._h, nsteps, checkpoint_every), "adjoint_forward")
+
+    def adjoint_backward(self, seed):
+        """Gradients (d/drho_0 [x][y][z][substrate], d/dtemp1, d/dtemp2 [agent][substrate]) of
+        sum(seed * rho_T) for the run recorded by ``adjoint_forward``."""
+        w = np.ascontiguousarray(seed, dtype=np.float64)
+        assert w.size == np.prod(self.shape)
+        na = getattr(self, "_nagents", 0)
+        g0 = np.empty(self.shape, dtype=np.float64)
+        g1 = np.empty((na, self.nsub), dtype=np.float64)
+        g2 = np.empty((na, self.nsub), dtype=np.float64)
+        self._check(self._lib.bfvmb_adjoint_backward(self._h, L.dptr(w), L.dptr(g0), L.dptr(g1) if na else None,
+                                                     L.dptr(g2) if na else None), "adjoint_backward")
+        return g0, g1, g2
 
     def autotune(self, reps: int = 3) -> np.ndarray:
         """Pick each sweep's launch shape by timing; returns [[wavefronts/CU, ring, us]] x 3."""

@@ -1,0 +1,346 @@
+// adjoint.hip -- reverse-mode derivative of nsteps of (sources/sinks, LOD step).
+//
+// Given a seed w = dL/drho_T, the reverse pass returns dL/drho_0 and dL/dtemp1, dL/dtemp2
+// of every agent for the run recorded by adjoint_forward.
+//
+// Forward step n:  rho <- AG(rho) (agents, BioFVM_basic_agent.cpp:279-300), then
+//                  rho <- D Z D Y D X D rho   (BioFVM_parallel.cpp:929-1392)
+// with D the Dirichlet overwrite (Jacobian M: 0 on active faces, 1 elsewhere) and X, Y, Z
+// the 1-D solves A^-1 of symmetric tridiagonal operators, so X^T = X etc.  The adjoint
+// step is therefore
+//     lam <- M X M Y M Z M lam   (the same sweep kernels, reverse order, Dirichlet value 0)
+//     lam <- AG^T lam            (per voxel, agents in reverse list order)
+// AG at one voxel composes r_a = (r_{a-1} + t1_a)/t2_a; its adjoint accumulates
+//     dL/dt1_a += lam_a / t2_a,  dL/dt2_a -= (lam_a / t2_a) r_a,  lam_{a-1} = lam_a / t2_a.
+// The states before each agent update are recomputed from checkpoints of the field taken
+// every `every` steps, one segment at a time, last segment first.
+#include <algorithm>
+#include <cstring>
+
+#include "bfvmb_internal.hpp"
+
+namespace bfvmb {
+namespace {
+
+// the first M of an adjoint step, on the two z faces (the z sweep's face lines and its
+// stores apply the rest)
+template <class T>
+__global__ __launch_bounds__(256) void k_zero_zfaces(T *f, const uint8_t *dact, int nx, int ny, int nz, int nsub)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nxy = (long long)nx * ny;
+    if (i >= nxy * nsub)
+        return;
+    const int s = (int)(i / nxy);
+    if (!dact[s])
+        return;
+    const long long xy = i - (long long)s * nxy;
+    T *p = f + (size_t)s * nxy * nz;
+    p[xy] = T(0);
+    p[(size_t)(nz - 1) * nxy + xy] = T(0);
+}
+
+// field values at the occupied voxels (the inputs of the agent update), [nsub][nseg]
+template <class T>
+__global__ __launch_bounds__(256) void k_gather(const T *f, const long long *seg_off, int nseg, int nsub, size_t nvox,
+                                                double *out)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)nseg * nsub)
+        return;
+    const int s = (int)(i / nseg);
+    const int k = (int)(i - (long long)s * nseg);
+    out[i] = (double)f[(size_t)s * nvox + seg_off[k]];
+}
+
+// adjoint of the agent update at every occupied voxel; rbuf is per-agent scratch
+template <class T>
+__global__ __launch_bounds__(256) void k_agents_adjoint(T *lam, const double *r0, const int *seg_start,
+                                                        const long long *seg_off, const double *t1, const double *t2,
+                                                        int nseg, int nact, int nsub, size_t nvox, double *rbuf,
+                                                        double *g1, double *g2)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)nseg * nsub)
+        return;
+    const int s = (int)(i / nseg);
+    const int k = (int)(i - (long long)s * nseg);
+    const double *a1 = t1 + (size_t)s * nact, *a2 = t2 + (size_t)s * nact;
+    double *rb = rbuf + (size_t)s * nact;
+    double r = r0[i];
+    const int a0 = seg_start[k], a1e = seg_start[k + 1];
+    for (int a = a0; a < a1e; ++a) {  // forward states r_a, as the agent kernel computes them
+        r = r + a1[a];
+        r = r / a2[a];
+        rb[a] = r;
+    }
+    T *p = lam + (size_t)s * nvox + seg_off[k];
+    double l = (double)*p;
+    for (int a = a1e - 1; a >= a0; --a) {
+        const double q = l / a2[a];
+        g1[(size_t)s * nact + a] += q;
+        g2[(size_t)s * nact + a] -= q * rb[a];
+        l = q;
+    }
+    *p = (T)l;
+}
+
+}  // namespace
+
+struct Tape {
+    int nsteps = 0, every = 1, nck = 0;
+    int start = 0;                 // buffer parity holding rho_0
+    void *ck = nullptr;            // nck fields: rho at steps 0, every, 2 every, ...
+    void *fin = nullptr;           // rho_T
+    unsigned long long epoch = 0;  // Group::epoch right after the forward run
+};
+
+namespace {
+
+void free_tape(Group *g)
+{
+    if (!g->tape)
+        return;
+    if (g->stream)
+        (void)hipStreamSynchronize(g->stream);
+    (void)hipFree(g->tape->ck);
+    (void)hipFree(g->tape->fin);
+    delete g->tape;
+    g->tape = nullptr;
+}
+
+// Scratch of one reverse pass, released on every exit.
+struct Scratch {
+    Group *g;
+    std::vector<void *> ptrs;
+    int alloc(void **p, size_t bytes)
+    {
+        int rc = hip_check(g, hipMalloc(p, bytes ? bytes : 8), "adjoint scratch");
+        if (!rc)
+            ptrs.push_back(*p);
+        return rc;
+    }
+    ~Scratch()
+    {
+        (void)hipStreamSynchronize(g->stream);
+        for (void *p : ptrs)
+            (void)hipFree(p);
+    }
+};
+
+template <class T>
+int backward_t(Group *g, const double *seed, double *grad_rho0, double *grad_t1, double *grad_t2)
+{
+    Tape &tp = *g->tape;
+    Domain &d = g->doms[0];
+    const size_t nel = d.nvox * d.nsub;
+    const size_t fbytes = nel * sizeof(T);
+    const int nseg = d.nseg, nact = d.nact, ns = d.nsub, every = tp.every, nsteps = tp.nsteps;
+    Scratch sc{g, {}};
+    int rc;
+    void *lam[2] = {nullptr, nullptr}, *zeros = nullptr, *stage = nullptr;
+    double *r0 = nullptr, *rbuf = nullptr, *g1 = nullptr, *g2 = nullptr;
+    if ((rc = sc.alloc(&lam[0], fbytes)) || (rc = sc.alloc(&lam[1], fbytes)) ||
+        (rc = sc.alloc(&stage, nel * sizeof(double))) || (rc = sc.alloc(&zeros, ns * sizeof(T))) ||
+        (rc = hip_check(g, hipMemsetAsync(zeros, 0, ns * sizeof(T), g->stream), "adjoint zeros")) ||
+        (rc = hip_check(g, hipMemcpyAsync(stage, seed, nel * sizeof(double), hipMemcpyHostToDevice, g->stream),
+                        "seed upload")) ||
+        (rc = launch_aos_to_soa(g, d, static_cast<const double *>(stage), lam[0])))
+        return rc;
+    if (nseg > 0) {
+        const size_t abytes = (size_t)nact * ns * sizeof(double);
+        if ((rc = sc.alloc((void **)&r0, (size_t)every * nseg * ns * sizeof(double))) ||
+            (rc = sc.alloc((void **)&rbuf, abytes)) || (rc = sc.alloc((void **)&g1, abytes)) ||
+            (rc = sc.alloc((void **)&g2, abytes)) ||
+            (rc = hip_check(g, hipMemsetAsync(g1, 0, abytes, g->stream), "gradient init")) ||
+            (rc = hip_check(g, hipMemsetAsync(g2, 0, abytes, g->stream), "gradient init")))
+            return rc;
+    }
+
+    // adjoint group: one domain on the lam buffers with the primal's tables and flags and
+    // Dirichlet value 0; it owns only its descriptor copies
+    Group adj;
+    adj.precision = g->precision;
+    adj.device = g->device;
+    adj.elem = g->elem;
+    adj.stream = g->stream;
+    adj.max_lds_per_block = g->max_lds_per_block;
+    adj.cu_count = g->cu_count;
+    for (int a = 0; a < 3; ++a)
+        adj.launch[a] = g->launch[a];
+    adj.doms.resize(1);
+    Domain &ad = adj.doms[0];
+    ad.nx = d.nx;
+    ad.ny = d.ny;
+    ad.nz = d.nz;
+    ad.nsub = d.nsub;
+    ad.dx = d.dx;
+    ad.nvox = d.nvox;
+    ad.field[0] = lam[0];
+    ad.field[1] = lam[1];
+    ad.coef = d.coef;
+    ad.have_coef = true;
+    ad.d_dval = zeros;
+    ad.d_dact = d.d_dact;
+    struct DescGuard {
+        Group &a;
+        ~DescGuard()
+        {
+            (void)hipFree(a.d_desc);
+            (void)hipFree(a.d_prefix);
+        }
+    } guard{adj};
+
+    auto field = [&](int parity) { return static_cast<T *>(d.field[parity]); };
+    auto ckp = [&](int j) { return static_cast<T *>(tp.ck) + (size_t)j * nel; };
+    const unsigned zgrid = (unsigned)(((long long)d.nx * d.ny * ns + 255) / 256);
+    const unsigned agrid = (unsigned)(((long long)nseg * ns + 255) / 256);
+    for (int seg = (nsteps - 1) / every; nsteps > 0 && seg >= 0; --seg) {
+        const int n0 = seg * every, n1 = std::min(nsteps, n0 + every);
+        // recompute the segment from its checkpoint, recording each step's agent inputs
+        if ((rc = hip_check(g, hipMemcpyAsync(field(tp.start), ckp(seg), fbytes, hipMemcpyDeviceToDevice, g->stream),
+                            "checkpoint restore")))
+            return rc;
+        g->cur = tp.start;
+        for (int n = n0; n < n1; ++n) {
+            if (nseg > 0)
+                hipLaunchKernelGGL(k_gather<T>, dim3(agrid), dim3(256), 0, g->stream, field(g->cur), d.seg_off, nseg, ns,
+                                   d.nvox, r0 + (size_t)(n - n0) * nseg * ns);
+            if ((rc = group_steps(g, 1, true, false)))
+                return rc;
+        }
+        // reverse steps, last first
+        for (int n = n1 - 1; n >= n0; --n) {
+            hipLaunchKernelGGL(k_zero_zfaces<T>, dim3(zgrid), dim3(256), 0, g->stream, static_cast<T *>(ad.field[adj.cur]),
+                               d.d_dact, d.nx, d.ny, d.nz, ns);
+            if ((rc = launch_sweep(&adj, 2)) || (rc = launch_sweep(&adj, 1)) || (rc = launch_sweep(&adj, 0)))
+                return set_error(g, rc, adj.err);
+            if (nseg > 0)
+                hipLaunchKernelGGL(k_agents_adjoint<T>, dim3(agrid), dim3(256), 0, g->stream,
+                                   static_cast<T *>(ad.field[adj.cur]), r0 + (size_t)(n - n0) * nseg * ns, d.seg_start,
+                                   d.seg_off, d.t1, d.t2, nseg, nact, ns, d.nvox, rbuf, g1, g2);
+            if ((rc = hip_check(g, hipGetLastError(), "adjoint launch")))
+                return rc;
+        }
+    }
+    // outputs
+    if (grad_rho0 &&
+        ((rc = launch_soa_to_aos(g, d, ad.field[adj.cur], static_cast<double *>(stage))) ||
+         (rc = hip_check(g, hipMemcpyAsync(grad_rho0, stage, nel * sizeof(double), hipMemcpyDeviceToHost, g->stream),
+                         "gradient download"))))
+        return rc;
+    std::vector<double> h1, h2;
+    if (nseg > 0 && (grad_t1 || grad_t2)) {
+        h1.resize((size_t)nact * ns);
+        h2.resize((size_t)nact * ns);
+        if ((rc = hip_check(g, hipMemcpyAsync(h1.data(), g1, h1.size() * sizeof(double), hipMemcpyDeviceToHost, g->stream),
+                            "gradient download")) ||
+            (rc = hip_check(g, hipMemcpyAsync(h2.data(), g2, h2.size() * sizeof(double), hipMemcpyDeviceToHost, g->stream),
+                            "gradient download")))
+            return rc;
+    }
+    // the context is left as the forward run left it: rho_T current, tape still valid
+    const int end = tp.start ^ (nsteps & 1);
+    if ((rc = hip_check(g, hipMemcpyAsync(field(end), tp.fin, fbytes, hipMemcpyDeviceToDevice, g->stream), "restore")) ||
+        (rc = hip_check(g, hipStreamSynchronize(g->stream), "adjoint sync")))
+        return rc;
+    g->cur = end;
+    g->epoch = tp.epoch;
+    const size_t nag = (size_t)d.agents_given;
+    if (grad_t1)
+        std::fill(grad_t1, grad_t1 + nag * ns, 0.0);
+    if (grad_t2)
+        std::fill(grad_t2, grad_t2 + nag * ns, 0.0);
+    if (!h1.empty())
+        for (int k = 0; k < nact; ++k) {
+            const size_t a = (size_t)d.agent_order[k];
+            for (int s = 0; s < ns; ++s) {
+                if (grad_t1)
+                    grad_t1[a * ns + s] = h1[(size_t)s * nact + k];
+                if (grad_t2)
+                    grad_t2[a * ns + s] = h2[(size_t)s * nact + k];
+            }
+        }
+    return BFVMB_OK;
+}
+
+}  // namespace
+
+void adjoint_free(Group *g)
+{
+    free_tape(g);
+}
+
+int adjoint_forward(Group *g, int nsteps, int every)
+{
+    if (g->doms.size() != 1 || nsteps < 0 || every < 1)
+        return set_error(g, BFVMB_ERR_ARG, "adjoint_forward: bad argument");
+    Domain &d = g->doms[0];
+    if (!d.have_coef)
+        return set_error(g, BFVMB_ERR_STATE, "set_coefficients has not been called");
+    free_tape(g);
+    Tape *tp = new Tape;
+    g->tape = tp;
+    tp->nsteps = nsteps;
+    tp->every = every;
+    tp->nck = std::max(1, (nsteps + every - 1) / every);
+    tp->start = g->cur;
+    const size_t fbytes = d.nvox * d.nsub * g->elem;
+    int rc;
+    if ((rc = hip_check(g, hipMalloc(&tp->ck, tp->nck * fbytes), "adjoint checkpoints")) ||
+        (rc = hip_check(g, hipMalloc(&tp->fin, fbytes), "adjoint final field"))) {
+        free_tape(g);
+        return rc;
+    }
+    char *ck = static_cast<char *>(tp->ck);
+    for (int n = 0; n < nsteps || n == 0; n += every) {
+        if ((rc = hip_check(g, hipMemcpyAsync(ck + (size_t)(n / every) * fbytes, d.field[g->cur], fbytes,
+                                              hipMemcpyDeviceToDevice, g->stream), "checkpoint")))
+            break;
+        // a segment is replayed as one captured graph when the context has a stream
+        if (nsteps > 0 && (rc = group_steps(g, std::min(every, nsteps - n), true, true)))
+            break;
+        if (nsteps == 0)
+            break;
+    }
+    if (!rc)
+        rc = hip_check(g, hipMemcpyAsync(tp->fin, d.field[g->cur], fbytes, hipMemcpyDeviceToDevice, g->stream),
+                       "final field");
+    if (!rc)
+        rc = hip_check(g, hipStreamSynchronize(g->stream), "adjoint forward sync");
+    if (rc) {
+        free_tape(g);
+        return rc;
+    }
+    tp->epoch = g->epoch;
+    return BFVMB_OK;
+}
+
+int adjoint_backward(Group *g, const double *seed, double *grad_rho0, double *grad_t1, double *grad_t2)
+{
+    if (!seed)
+        return set_error(g, BFVMB_ERR_ARG, "adjoint_backward: seed is NULL");
+    if (!g->tape || g->tape->epoch != g->epoch)
+        return set_error(g, BFVMB_ERR_STATE,
+                         "adjoint_backward: no adjoint_forward record for the current field, coefficients and agents");
+    if (g->precision == BFVMB_FP64)
+        return backward_t<double>(g, seed, grad_rho0, grad_t1, grad_t2);
+    return backward_t<float>(g, seed, grad_rho0, grad_t1, grad_t2);
+}
+
+}  // namespace bfvmb
+
+extern "C" int bfvmb_adjoint_forward(bfvmb_ctx *ctx, int nsteps, int checkpoint_every)
+{
+    if (!ctx)
+        return BFVMB_ERR_ARG;
+    return bfvmb::adjoint_forward(ctx, nsteps, checkpoint_every);
+}
+
+extern "C" int bfvmb_adjoint_backward(bfvmb_ctx *ctx, const double *seed, double *grad_rho0, double *grad_temp1,
+                                      double *grad_temp2)
+{
+    if (!ctx)
+        return BFVMB_ERR_ARG;
+    return bfvmb::adjoint_backward(ctx, seed, grad_rho0, grad_temp1, grad_temp2);
+}

@@ -72,6 +72,8 @@ struct Domain {
     int *seg_start = nullptr;
     long long *seg_off = nullptr;
     double *t1 = nullptr, *t2 = nullptr;
+    std::vector<int> agent_order;  // sorted position -> index in the caller's agent list
+    int agents_given = 0;          // length of the caller's agent list
 };
 
 // Offsets inside a domain's coefficient block, in doubles.
@@ -101,6 +103,10 @@ struct Group {
     // captured step graphs, keyed by (steps, parity, with_agents)
     std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
     std::string err;
+    // bumped by every change of field, coefficients, Dirichlet data or agents; an adjoint
+    // tape is valid only in the epoch it was recorded in
+    unsigned long long epoch = 0;
+    struct Tape *tape = nullptr;  // adjoint checkpoints (adjoint.hip)
 };
 
 // Launchers (lod.hip, field.hip): one kernel over all domains of the group.
@@ -134,6 +140,10 @@ int group_set_stream(Group *g, void *stream);
 int group_launch_phase(Group *g, int phase, int count);
 int group_set_launch(Group *g, int sweep, long long grid, int bpc, int ring);
 int group_autotune(Group *g, int reps, int *chosen);
+// Adjoint (adjoint.hip): forward run recording checkpoints, then the reverse pass.
+int adjoint_forward(Group *g, int nsteps, int every);
+int adjoint_backward(Group *g, const double *seed, double *grad_rho0, double *grad_t1, double *grad_t2);
+void adjoint_free(Group *g);
 
 }  // namespace bfvmb
 

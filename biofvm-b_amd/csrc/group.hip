@@ -233,6 +233,7 @@ void group_free(Group *g)
         (void)hipFree(d.d_dval);
         (void)hipFree(d.d_dact);
     }
+    adjoint_free(g);
     g->doms.clear();
     (void)hipFree(g->d_desc);
     (void)hipFree(g->d_prefix);
@@ -293,6 +294,7 @@ int domain_set_coefficients(Group *g, int i, const double *D, const double *deca
     d.decay.assign(decay, decay + ns);
     d.dt = dt;
     d.have_coef = true;
+    g->epoch++;
     return BFVMB_OK;
 }
 
@@ -304,6 +306,7 @@ int domain_set_dirichlet(Group *g, int i, const double *value, const uint8_t *ac
     d.dval.assign(value, value + d.nsub);
     for (int s = 0; s < d.nsub; ++s)
         d.dact[s] = active[s] ? 1 : 0;
+    g->epoch++;
     return upload_dirichlet(g, d);
 }
 
@@ -327,6 +330,7 @@ int domain_upload(Group *g, int i, const double *rho)
     Domain &d = g->doms[i];
     const size_t bytes = d.nvox * d.nsub * sizeof(double);
     int rc;
+    g->epoch++;
     if ((rc = ensure_stage(g, bytes)) ||
         (rc = hip_check(g, hipMemcpyAsync(g->stage, rho, bytes, hipMemcpyHostToDevice, g->stream), "density upload")) ||
         (rc = launch_aos_to_soa(g, d, static_cast<const double *>(g->stage), d.field[g->cur])))
@@ -389,7 +393,10 @@ int domain_set_agents(Group *g, int i, int n, const int *voxel, const double *te
     if (rc)
         return rc;
     free_agents(d);
+    d.agent_order = order;
+    d.agents_given = n;
     g->desc_dirty = true;
+    g->epoch++;
     if (nact == 0)
         return BFVMB_OK;
     d.nseg = (int)seg_off.size();
@@ -431,6 +438,7 @@ int group_steps(Group *g, int nsteps, bool with_agents, bool use_graph)
     int rc = sync_descriptors(g);
     if (rc || nsteps == 0)
         return rc;
+    g->epoch++;
     if (!use_graph || g->stream == nullptr)
         return eager_steps(g, nsteps, with_agents);
     // the whole run as one graph: nsteps x (agents, x, y, z) kernel nodes, replayed as one launch
@@ -469,6 +477,7 @@ int group_launch_phase(Group *g, int phase, int count)
         for (const Domain &d : g->doms)
             if (!d.have_coef)
                 return set_error(g, BFVMB_ERR_STATE, "set_coefficients has not been called for every domain");
+    g->epoch++;
     for (int i = 0; i < count; ++i) {
         int rc = phase == 0 ? launch_agents(g) : launch_sweep(g, phase - 1);
         if (rc)

@@ -122,6 +122,25 @@ int bfvmb_set_launch(bfvmb_ctx *ctx, int sweep, int grid_blocks, int lds_blocks_
  * (may be NULL) receives 3 x {wavefronts per CU, ring, microseconds per launch}. */
 int bfvmb_autotune(bfvmb_ctx *ctx, int reps, int *chosen);
 
+/* ---- Adjoint.  The reference has no adjoint: these differentiate its time loop
+ * (complete_diffusion_demo.cpp:489-493) through the LOD step (BioFVM_parallel.cpp:929-1392)
+ * and the agent update (BioFVM_basic_agent.cpp:279-300), with the agent tables of
+ * bfvmb_set_agents / bfvmb_set_agent_rates held fixed.
+ *
+ * bfvmb_adjoint_forward: `nsteps` of bfvmb_run from the current field rho_0 (same results),
+ * recording a field checkpoint every `checkpoint_every` steps; the field is rho_T after it.
+ * bfvmb_adjoint_backward: for the recorded run and a seed w = dL/drho_T (voxel-major
+ * [voxel][substrate], like the densities), the gradients dL/drho_0 (same layout) and
+ * dL/dtemp1, dL/dtemp2 ([agent][substrate] in the order given to set_agents, zero for
+ * inactive agents).  Any gradient may be NULL.  It recomputes the forward states segment
+ * by segment from the checkpoints and leaves the context as the forward run left it, so it
+ * may be called again with another seed.  Any change of field, coefficients, Dirichlet
+ * data or agents after the forward run (including further steps) makes it return
+ * BFVMB_ERR_STATE. */
+int bfvmb_adjoint_forward(bfvmb_ctx *ctx, int nsteps, int checkpoint_every);
+int bfvmb_adjoint_backward(bfvmb_ctx *ctx, const double *seed, double *grad_rho0, double *grad_temp1,
+                           double *grad_temp2);
+
 /* LDS bytes per workgroup and CU count of HIP device `device`. */
 int bfvmb_device_info(int device, int *lds_per_block, int *cu_count);
 

@@ -133,12 +133,15 @@ def sources_sinks(rho, voxel, active, t1, t2):
     lib().orc_sources_sinks(len(v), nsub, pv, pa, p1, p2, rho.ctypes.data_as(_dp))
 
 
-def run(case, nsteps=None, rho=None, with_agents=None):
-    """Run ``case`` with the C restatement; returns the AoS field after ``nsteps``."""
+def run(case, nsteps=None, rho=None, with_agents=None, tables=None):
+    """Run ``case`` with the C restatement; returns the AoS field after ``nsteps``.
+    ``tables`` = (temp1, temp2) replaces the agent constants derived from the case."""
     nsteps = case.nsteps if nsteps is None else nsteps
     with_agents = case.with_agents if with_agents is None else with_agents
     rho = np.array(case.init if rho is None else rho, dtype=np.float64, order="C", copy=True)
     voxel, t1, t2 = agent_tables(case)
+    if tables is not None:
+        t1, t2 = (np.ascontiguousarray(t, dtype=np.float64).reshape(t1.shape) for t in tables)
     na = case.nagents if with_agents else 0
     active = np.ones(max(na, 1), dtype=np.uint8)
     D, pD = _d(case.D)

@@ -1,0 +1,199 @@
+"""Adjoint of the time loop (bfvmb_adjoint_forward / bfvmb_adjoint_backward) for the loss
+L = sum((rho_T - target)^2) after nsteps of (sources/sinks, LOD step).  Gradients are checked against central finite differences
+of the oracle's forward run (the reference's arithmetic), so a wrong transpose, a missed
+Dirichlet face or a wrong agent order shows up as a mismatch with the reference's own map.
+
+Tolerances: FP64 directional derivatives within 1e-6 relative (L is quadratic in rho_0,
+so central differences are exact up to rounding; in temp1/temp2 the truncation error at a
+relative step of 1e-5 is ~1e-10).  FP32 storage: gradients within 1e-4 of FP64 (norm-wise).
+"""
+import numpy as np
+import pytest
+
+import casefile as cf
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(nsteps=7, seed=3, **kw):
+    args = dict(seed=seed, nagents=40, nsteps=nsteps, shared_voxels=True, outside=1, dactive=[1, 0, 1])
+    args.update(kw)
+    return cf.make_case(9, 8, 11, 3, **args)
+
+
+def _solver(c, precision="fp64", tables=None):
+    from biofvmb import LODSolver
+    s = LODSolver(c.nx, c.ny, c.nz, c.nsub, c.dx, precision)
+    s.set_coefficients(c.D, c.decay, c.dt)
+    dv, da = cf.effective_dirichlet(c)
+    s.set_dirichlet(dv, da)
+    s.upload(c.init)
+    voxel, t1, t2 = orc.agent_tables(c)
+    if tables is not None:
+        t1, t2 = tables
+    s.set_agents(voxel, t1, t2)
+    return s
+
+
+def _loss(c, target, rho=None, tables=None):
+    r = orc.run(c, rho=rho, tables=tables)
+    return float(np.sum((r - target) ** 2))
+
+
+def _target(c, seed=11):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(0.0, 2.0, c.init.shape)
+
+
+def _fd_dir(f, x, v, eps):
+    return (f(x + eps * v) - f(x - eps * v)) / (2 * eps)
+
+
+def _adjoint(s, nsteps, tgt, every=10):
+    """Squared loss and its gradients through the C ABI: forward, seed 2 (rho_T - target),
+    backward."""
+    s.adjoint_forward(nsteps, every)
+    rT = s.download()
+    g0, g1, g2 = s.adjoint_backward(2.0 * (rT - tgt))
+    return float(np.sum((rT - tgt) ** 2)), g0, g1, g2, rT
+
+
+def test_loss_and_field_restored():
+    c = _case()
+    tgt = _target(c)
+    print("[gpu] adjoint loss 9x8x11x3", flush=True)
+    s = _solver(c)
+    L0, g0, g1, g2, rT = _adjoint(s, c.nsteps, tgt, every=3)
+    assert np.array_equal(rT, orc.run(c))  # the recorded forward run is the reference's map
+    assert L0 == _loss(c, tgt)
+    # backward leaves the context at rho_T with the record valid: a second seed works
+    assert np.array_equal(s.download(), rT)
+    h0, h1, h2 = s.adjoint_backward(2.0 * (rT - tgt))
+    assert np.array_equal(h0, g0) and np.array_equal(h1, g1) and np.array_equal(h2, g2)
+    # linear in the seed
+    k0, _, _ = s.adjoint_backward(-3.0 * (rT - tgt))
+    assert np.allclose(k0, -1.5 * g0, rtol=1e-14, atol=0)
+    # more steps invalidate the record; the context keeps stepping exactly
+    s.run(2)
+    with pytest.raises(Exception, match="adjoint_backward"):
+        s.adjoint_backward(rT)
+    assert np.array_equal(s.download(), orc.run(c, nsteps=c.nsteps + 2))
+    s.close()
+
+
+def test_grad_rho0_matches_finite_differences():
+    c = _case()
+    tgt = _target(c)
+    s = _solver(c)
+    L, g0, _, _, _ = _adjoint(s, c.nsteps, tgt, every=3)
+    s.close()
+    rng = np.random.default_rng(5)
+    f = lambda r: _loss(c, tgt, rho=r)  # noqa: E731
+    for k in range(4):
+        v = rng.standard_normal(c.init.shape)
+        fd = _fd_dir(f, c.init, v, 1e-3)
+        ad = float(np.sum(g0 * v))
+        print(f"dir {k}: fd {fd:.12e} adjoint {ad:.12e}")
+        assert abs(fd - ad) <= 1e-6 * abs(fd)
+    # single entries: an active Dirichlet face (gradient exactly 0 for an active substrate),
+    # an interior voxel, an occupied voxel
+    voxel, _, _ = orc.agent_tables(c)
+    idx = np.unravel_index(int(voxel[0]), c.init.shape[:3])
+    assert np.all(g0[0, :, :, 0] == 0.0) and np.all(g0[:, :, -1, 2] == 0.0)
+    for pt in [(4, 3, 5, 1), (4, 3, 5, 0), tuple(idx) + (2,)]:
+        e = np.zeros(c.init.shape)
+        e[pt] = 1.0
+        fd = _fd_dir(f, c.init, e, 1e-3)
+        assert abs(fd - g0[pt]) <= 1e-6 * max(abs(fd), 1e-12), (pt, fd, g0[pt])
+
+
+def test_grad_agent_tables_match_finite_differences():
+    c = _case()
+    tgt = _target(c)
+    voxel, t1, t2 = orc.agent_tables(c)
+    s = _solver(c)
+    L, _, g1, g2, _ = _adjoint(s, c.nsteps, tgt, every=10)
+    s.close()
+    rng = np.random.default_rng(8)
+    f1 = lambda x: _loss(c, tgt, tables=(x, t2))  # noqa: E731
+    f2 = lambda x: _loss(c, tgt, tables=(t1, x))  # noqa: E731
+    for k in range(3):
+        v = rng.standard_normal(t1.shape) * np.abs(t1).max()
+        fd = _fd_dir(f1, t1, v, 1e-5)
+        ad = float(np.sum(g1 * v))
+        print(f"temp1 dir {k}: fd {fd:.10e} adjoint {ad:.10e}")
+        assert abs(fd - ad) <= 1e-6 * abs(fd)
+        v = rng.standard_normal(t2.shape) * np.abs(t2).max()
+        fd = _fd_dir(f2, t2, v, 1e-5)
+        ad = float(np.sum(g2 * v))
+        print(f"temp2 dir {k}: fd {fd:.10e} adjoint {ad:.10e}")
+        assert abs(fd - ad) <= 1e-6 * abs(fd)
+    # agents sharing a voxel: each gets its own gradient, in the caller's order
+    for a in (0, 1, c.nagents - 1):
+        for sub in range(c.nsub):
+            e = np.zeros(t1.shape)
+            e[a, sub] = max(abs(t1[a, sub]), 1.0)
+            fd = _fd_dir(f1, t1, e, 1e-5)
+            assert abs(fd - g1[a, sub] * e[a, sub]) <= 1e-6 * max(abs(fd), 1e-9), (a, sub)
+
+
+def test_checkpoint_interval_does_not_change_results():
+    c = _case(nsteps=12)
+    tgt = _target(c)
+    outs = []
+    for k in (1, 5, 12, 40):
+        s = _solver(c)
+        outs.append(_adjoint(s, c.nsteps, tgt, every=k))
+        s.close()
+    for o in outs[1:]:
+        for a, b in zip(o, outs[0]):
+            assert np.array_equal(a, b)
+
+
+def test_inactive_agents_get_zero_gradient():
+    c = _case()
+    tgt = _target(c)
+    voxel, t1, t2 = orc.agent_tables(c)
+    from biofvmb import LODSolver
+    s = LODSolver(c.nx, c.ny, c.nz, c.nsub, c.dx, "fp64")
+    s.set_coefficients(c.D, c.decay, c.dt)
+    s.set_dirichlet(*cf.effective_dirichlet(c))
+    s.upload(c.init)
+    act = np.ones(c.nagents, dtype=np.uint8)
+    act[::3] = 0
+    s.set_agents(voxel, t1, t2, active=act)
+    _, _, g1, g2, _ = _adjoint(s, c.nsteps, tgt)
+    s.close()
+    assert np.all(g1[::3] == 0.0) and np.all(g2[::3] == 0.0)
+    assert np.any(g1[1::3] != 0.0)
+
+
+def test_zero_steps_and_no_agents():
+    c = _case(nagents=0)
+    tgt = _target(c)
+    s = _solver(c)
+    L, g0, g1, g2, _ = _adjoint(s, 0, tgt)
+    assert np.array_equal(g0, 2.0 * (c.init - tgt)) and g1.size == 0
+    s.upload(c.init)
+    L, g0, _, _, _ = _adjoint(s, 3, tgt, every=2)
+    v = np.random.default_rng(2).standard_normal(c.init.shape)
+    fd = _fd_dir(lambda r: float(np.sum((orc.run(c, nsteps=3, rho=r) - tgt) ** 2)), c.init, v, 1e-3)
+    assert abs(fd - float(np.sum(g0 * v))) <= 1e-6 * abs(fd)
+    s.close()
+
+
+def test_fp32_gradients_close_to_fp64():
+    c = _case()
+    tgt = _target(c)
+    s64 = _solver(c, "fp64")
+    s32 = _solver(c, "fp32")
+    L64, a0, a1, a2, _ = _adjoint(s64, c.nsteps, tgt)
+    L32, b0, b1, b2, _ = _adjoint(s32, c.nsteps, tgt)
+    s64.close()
+    s32.close()
+    assert abs(L32 - L64) <= 1e-5 * L64
+    for a, b in ((a0, b0), (a1, b1), (a2, b2)):
+        err = np.linalg.norm(a - b) / np.linalg.norm(a)
+        print(f"fp32 gradient relative error {err:.2e}")
+        assert err <= 1e-4
