@@ -69,6 +69,11 @@ class LODSolver:
     def nsub(self) -> int:
         return self.shape[3]
 
+    @property
+    def handle(self) -> int:
+        """Address of the bfvmb_ctx (for the torch operators)."""
+        return int(self._h.value or 0)
+
     # -- set-up ---------------------------------------------------------------------------
     def set_stream(self, stream_ptr: int | None) -> None:
         self._check(self._lib.bfvmb_set_stream(self._h, stream_ptr), "set_stream")

@@ -5,7 +5,8 @@ Dirichlet face or a wrong agent order shows up as a mismatch with the reference'
 
 Tolerances: FP64 directional derivatives within 1e-6 relative (L is quadratic in rho_0,
 so central differences are exact up to rounding; in temp1/temp2 the truncation error at a
-relative step of 1e-5 is ~1e-10).  FP32 storage: gradients within 1e-4 of FP64 (norm-wise).
+relative step of 1e-5 is ~1e-10; rho_T is affine in temp1, so those differences are exact
+up to rounding too).  FP32 storage: gradients within 1e-4 of FP64 (norm-wise).
 """
 import numpy as np
 import pytest
@@ -46,8 +47,12 @@ def _target(c, seed=11):
     return rng.uniform(0.0, 2.0, c.init.shape)
 
 
-def _fd_dir(f, x, v, eps):
-    return (f(x + eps * v) - f(x - eps * v)) / (2 * eps)
+def _fd_dir(f, x, v, eps, tgt):
+    """Central difference of L = sum((f(x) - tgt)^2) along v, f returning rho_T; the
+    difference of the two losses is formed as sum((a - b)(a + b - 2 tgt)), which does not
+    cancel."""
+    a, b = f(x + eps * v), f(x - eps * v)
+    return float(np.sum((a - b) * (a + b - 2.0 * tgt))) / (2 * eps)
 
 
 def _adjoint(s, nsteps, tgt, every=10):
@@ -89,10 +94,10 @@ def test_grad_rho0_matches_finite_differences():
     L, g0, _, _, _ = _adjoint(s, c.nsteps, tgt, every=3)
     s.close()
     rng = np.random.default_rng(5)
-    f = lambda r: _loss(c, tgt, rho=r)  # noqa: E731
+    f = lambda r: orc.run(c, rho=r)  # noqa: E731
     for k in range(4):
         v = rng.standard_normal(c.init.shape)
-        fd = _fd_dir(f, c.init, v, 1e-3)
+        fd = _fd_dir(f, c.init, v, 1e-3, tgt)
         ad = float(np.sum(g0 * v))
         print(f"dir {k}: fd {fd:.12e} adjoint {ad:.12e}")
         assert abs(fd - ad) <= 1e-6 * abs(fd)
@@ -104,7 +109,7 @@ def test_grad_rho0_matches_finite_differences():
     for pt in [(4, 3, 5, 1), (4, 3, 5, 0), tuple(idx) + (2,)]:
         e = np.zeros(c.init.shape)
         e[pt] = 1.0
-        fd = _fd_dir(f, c.init, e, 1e-3)
+        fd = _fd_dir(f, c.init, e, 1e-3, tgt)
         assert abs(fd - g0[pt]) <= 1e-6 * max(abs(fd), 1e-12), (pt, fd, g0[pt])
 
 
@@ -116,16 +121,16 @@ def test_grad_agent_tables_match_finite_differences():
     L, _, g1, g2, _ = _adjoint(s, c.nsteps, tgt, every=10)
     s.close()
     rng = np.random.default_rng(8)
-    f1 = lambda x: _loss(c, tgt, tables=(x, t2))  # noqa: E731
-    f2 = lambda x: _loss(c, tgt, tables=(t1, x))  # noqa: E731
+    f1 = lambda x: orc.run(c, tables=(x, t2))  # noqa: E731
+    f2 = lambda x: orc.run(c, tables=(t1, x))  # noqa: E731
     for k in range(3):
         v = rng.standard_normal(t1.shape) * np.abs(t1).max()
-        fd = _fd_dir(f1, t1, v, 1e-5)
+        fd = _fd_dir(f1, t1, v, 0.5, tgt)  # rho_T is affine in temp1: L is quadratic
         ad = float(np.sum(g1 * v))
         print(f"temp1 dir {k}: fd {fd:.10e} adjoint {ad:.10e}")
         assert abs(fd - ad) <= 1e-6 * abs(fd)
         v = rng.standard_normal(t2.shape) * np.abs(t2).max()
-        fd = _fd_dir(f2, t2, v, 1e-5)
+        fd = _fd_dir(f2, t2, v, 1e-5, tgt)
         ad = float(np.sum(g2 * v))
         print(f"temp2 dir {k}: fd {fd:.10e} adjoint {ad:.10e}")
         assert abs(fd - ad) <= 1e-6 * abs(fd)
@@ -134,7 +139,7 @@ def test_grad_agent_tables_match_finite_differences():
         for sub in range(c.nsub):
             e = np.zeros(t1.shape)
             e[a, sub] = max(abs(t1[a, sub]), 1.0)
-            fd = _fd_dir(f1, t1, e, 1e-5)
+            fd = _fd_dir(f1, t1, e, 0.5, tgt)
             assert abs(fd - g1[a, sub] * e[a, sub]) <= 1e-6 * max(abs(fd), 1e-9), (a, sub)
 
 
@@ -178,7 +183,7 @@ def test_zero_steps_and_no_agents():
     s.upload(c.init)
     L, g0, _, _, _ = _adjoint(s, 3, tgt, every=2)
     v = np.random.default_rng(2).standard_normal(c.init.shape)
-    fd = _fd_dir(lambda r: float(np.sum((orc.run(c, nsteps=3, rho=r) - tgt) ** 2)), c.init, v, 1e-3)
+    fd = _fd_dir(lambda r: orc.run(c, nsteps=3, rho=r), c.init, v, 1e-3, tgt)
     assert abs(fd - float(np.sum(g0 * v))) <= 1e-6 * abs(fd)
     s.close()
 
@@ -197,3 +202,63 @@ def test_fp32_gradients_close_to_fp64():
         err = np.linalg.norm(a - b) / np.linalg.norm(a)
         print(f"fp32 gradient relative error {err:.2e}")
         assert err <= 1e-4
+
+
+def _rates_case(nsteps=20):
+    return cf.make_case(20, 18, 16, 2, seed=21, nagents=300, nsteps=nsteps, shared_voxels=True,
+                        D=[1e5, 1e3], decay=[0.1, 0.01], dactive=[1, 0], dvalue=[38.0, 0.0])
+
+
+def test_torch_autograd_field_and_rates():
+    """torch.ops.biofvmb.lod_run + agent_tables: forward equals the reference run; the
+    backward gradients in rho_0 and in the secretion/uptake rates match central differences
+    of the oracle's map on 32 random coordinates (checkpoint every 10 steps)."""
+    import torch
+    from biofvmb import autograd as ag
+    c = _rates_case()
+    dV = (c.dx * c.dx) * c.dx
+    voxel, _, _ = orc.agent_tables(c)
+    # target: the reference run with rates perturbed by up to 10%
+    rng = np.random.default_rng(31)
+    pS, pU = c.S * rng.uniform(0.9, 1.1, c.S.shape), c.U * rng.uniform(0.9, 1.1, c.U.shape)
+    tgt = orc.run(c, tables=orc.agent_constants(c.dt, c.volume, dV, pS, pU, c.T))
+    s = _solver(c)
+    dev = "cuda"
+    rho0 = torch.tensor(c.init, device=dev, requires_grad=True)
+    S = torch.tensor(c.S, device=dev, requires_grad=True)
+    U = torch.tensor(c.U, device=dev, requires_grad=True)
+    t1, t2 = ag.agent_tables(torch.tensor(c.volume, device=dev), S, U, torch.tensor(c.T, device=dev), c.dt, dV)
+    rT = ag.lod_run(s, rho0, torch.tensor(voxel, device=dev), t1, t2, c.nsteps, 10)
+    assert np.array_equal(rT.detach().cpu().numpy(), orc.run(c))
+    loss = ((rT - torch.tensor(tgt, device=dev)) ** 2).sum()
+    loss.backward()
+    gr, gS, gU = (x.grad.cpu().numpy() for x in (rho0, S, U))
+    s.close()
+
+    def field_of_rates(Sx, Ux):
+        return orc.run(c, tables=orc.agent_constants(c.dt, c.volume, dV, Sx, Ux, c.T))
+
+    worst = 0.0
+    coords = [("rho", tuple(rng.integers(0, n) for n in c.init.shape)) for _ in range(16)]
+    coords += [("S", (int(rng.integers(0, c.nagents)), int(rng.integers(0, c.nsub)))) for _ in range(8)]
+    coords += [("U", (int(rng.integers(0, c.nagents)), int(rng.integers(0, c.nsub)))) for _ in range(8)]
+    for kind, idx in coords:
+        if kind == "rho":
+            e = np.zeros(c.init.shape)
+            e[idx] = 1.0
+            fd = _fd_dir(lambda r: orc.run(c, rho=r), c.init, e, 1e-3, tgt)
+            ad = gr[idx]
+        else:
+            x0 = c.S if kind == "S" else c.U
+            e = np.zeros(x0.shape)
+            e[idx] = max(abs(x0[idx]), 1e-3)
+            f = (lambda x: field_of_rates(x, c.U)) if kind == "S" else (lambda x: field_of_rates(c.S, x))
+            # Richardson-extrapolated central differences (truncation O(h^4)) at relative
+            # steps 4e-3 / 8e-3: the gradients of single rates span four decades, and
+            # smaller steps leave the rounding of rho_T above 1e-6 for the smallest
+            fd = (4 * _fd_dir(f, x0, e, 4e-3, tgt) - _fd_dir(f, x0, e, 8e-3, tgt)) / 3
+            ad = (gS if kind == "S" else gU)[idx] * e[idx]
+        rel = abs(fd - ad) / max(abs(fd), 1e-300)
+        worst = max(worst, rel if abs(fd) > 1e-12 else 0.0)
+        assert abs(fd - ad) <= 1e-6 * abs(fd) or fd == ad == 0.0, (kind, idx, fd, ad)
+    print(f"32 coordinates: worst relative error {worst:.2e}")
