@@ -14,7 +14,7 @@ PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "lib", "libbiofvmb.so")
 HEADER_PATH = os.path.join(os.path.dirname(PKG_DIR), "include", "biofvmb.h")
 
-OK, ERR_ARG, ERR_STATE, ERR_HIP, ERR_NOMEM, ERR_NODEV = 0, 1, 2, 3, 4, 5
+OK, ERR_ARG, ERR_STATE, ERR_HIP, ERR_NOMEM, ERR_NODEV, ERR_IO = 0, 1, 2, 3, 4, 5, 6
 FP64, FP32 = 0, 1
 
 _c = ctypes
@@ -46,6 +46,11 @@ SIGNATURES = {
     "bfvmb_run_graph": (_c.c_int, [_vp, _c.c_int]),
     "bfvmb_autotune": (_c.c_int, [_vp, _c.c_int, _ip]),
     "bfvmb_adjoint_forward": (_c.c_int, [_vp, _c.c_int, _c.c_int]),
+    "bfvmb_write_matlab": (_c.c_int, [_vp, _c.c_char_p, _dp]),
+    "bfvmb_set_bulk_rates": (_c.c_int, [_vp, _dp, _dp, _dp]),
+    "bfvmb_simulate_bulk_sources_and_sinks": (_c.c_int, [_vp, _c.c_double]),
+    "bfvmb_batch_set_bulk_rates": (_c.c_int, [_vp, _c.c_int, _dp, _dp, _dp]),
+    "bfvmb_read_matlab": (_c.c_int, [_vp, _c.c_char_p, _dp]),
     "bfvmb_adjoint_backward": (_c.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "bfvmb_device_info": (_c.c_int, [_c.c_int, _ip, _ip]),
     "bfvmb_density_device_ptr": (_c.c_int, [_vp, _c.POINTER(_vp)]),

@@ -119,6 +119,18 @@ class LODSolver:
         self._check(self._lib.bfvmb_set_agent_rates(self._h, n, L.iptr(v), L.dptr(vol), L.dptr(S), L.dptr(U), L.dptr(T),
                                                     None if a is None else L.bptr(a), float(dt)), "set_agent_rates")
 
+    def set_bulk_rates(self, supply=None, target=None, uptake=None) -> None:
+        """Per-voxel bulk supply S, target T, uptake U, each [x][y][z][substrate] (None = 0);
+        all None removes the bulk phase (bfvmb_set_bulk_rates)."""
+        keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(self.shape)
+                for a in (supply, target, uptake)]
+        self._check(self._lib.bfvmb_set_bulk_rates(self._h, *[None if k is None else L.dptr(k) for k in keep]),
+                    "set_bulk_rates")
+
+    def simulate_bulk_sources_and_sinks(self, dt: float) -> None:
+        self._check(self._lib.bfvmb_simulate_bulk_sources_and_sinks(self._h, float(dt)),
+                    "simulate_bulk_sources_and_sinks")
+
     def voxel_indices(self, pos, bbox_min=(0.0, 0.0, 0.0)) -> np.ndarray:
         """Agent voxel per position (-1 outside the bounding box), reference numbering."""
         return voxel_indices(self.shape[:3], self.dx, pos, bbox_min)
@@ -137,6 +149,16 @@ class LODSolver:
     def run(self, nsteps: int, graph: bool = False) -> None:
         fn = self._lib.bfvmb_run_graph if graph else self._lib.bfvmb_run
         self._check(fn(self._h, nsteps), "run")
+
+    def write_matlab(self, filename: str, bbox_min=(0.0, 0.0, 0.0)) -> None:
+        """Microenvironment::write_to_matlab of the resident field (bfvmb_write_matlab)."""
+        lo = np.ascontiguousarray(bbox_min, dtype=np.float64)
+        self._check(self._lib.bfvmb_write_matlab(self._h, str(filename).encode(), L.dptr(lo)), "write_matlab")
+
+    def read_matlab(self, filename: str, bbox_min=(0.0, 0.0, 0.0)) -> None:
+        """Load a multiscale_microenvironment MAT v4 file into the field (bfvmb_read_matlab)."""
+        lo = np.ascontiguousarray(bbox_min, dtype=np.float64)
+        self._check(self._lib.bfvmb_read_matlab(self._h, str(filename).encode(), L.dptr(lo)), "read_matlab")
 
     def adjoint_forward(self, nsteps: int, checkpoint_every: int = 10) -> None:
         """``nsteps`` of ``run`` recording checkpoints for ``adjoint_backward``."""
@@ -216,6 +238,12 @@ class Batch:
         lam = np.ascontiguousarray(decay, dtype=np.float64)
         self._check(self._lib.bfvmb_batch_set_coefficients(self._h, i, L.dptr(D), L.dptr(lam), float(dt)),
                     "set_coefficients")
+
+    def set_bulk_rates(self, i, supply=None, target=None, uptake=None):
+        keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(self.shapes[i])
+                for a in (supply, target, uptake)]
+        self._check(self._lib.bfvmb_batch_set_bulk_rates(self._h, i, *[None if k is None else L.dptr(k)
+                                                                          for k in keep]), "set_bulk_rates")
 
     def set_dirichlet(self, i, value, active):
         ns = self.shapes[i][3]

@@ -42,6 +42,9 @@ struct DomainDesc {
     const int *seg_start;        // agents: [nseg+1] offsets into the voxel-sorted list
     const long long *seg_off;    // agents: [nseg] voxel offset z*nx*ny + y*nx + x
     const double *t1, *t2;       // agents: [nsub][nact] temp1 / temp2, sorted order
+    const double *bulk_ts;       // bulk: [nsub][nz][ny][nx] target*supply (nullptr: none)
+    const double *bulk_us;       // bulk: [nsub][nz][ny][nx] uptake+supply
+    double dt;                   // time step frozen by set_coefficients
     int nseg, nact;
     int nx, ny, nz, nsub;
 };
@@ -74,6 +77,7 @@ struct Domain {
     double *t1 = nullptr, *t2 = nullptr;
     std::vector<int> agent_order;  // sorted position -> index in the caller's agent list
     int agents_given = 0;          // length of the caller's agent list
+    double *bulk = nullptr;        // bulk sources/sinks: T*S then U+S, each [nsub][nz][ny][nx]
 };
 
 // Offsets inside a domain's coefficient block, in doubles.
@@ -93,10 +97,11 @@ struct Group {
     LaunchShape launch[3];
     size_t max_lds_per_block = 0;
     int cu_count = 0;
-    // device descriptors and per-phase prefix sums (0 = agent work items, 1..3 = x,y,z tiles)
+    // device descriptors and per-phase prefix sums (0 = agent work items, 1..3 = x,y,z
+    // tiles, 4 = bulk elements)
     void *d_desc = nullptr;
     long long *d_prefix = nullptr;
-    std::vector<long long> prefix[4];
+    std::vector<long long> prefix[5];
     bool desc_dirty = true;
     void *stage = nullptr;  // staging for AoS<->SoA transfers (double)
     size_t stage_bytes = 0;
@@ -112,7 +117,10 @@ struct Group {
 // Launchers (lod.hip, field.hip): one kernel over all domains of the group.
 int launch_sweep(Group *g, int axis);   // field[cur] -> field[cur^1] for every domain
 int launch_agents(Group *g);            // in place on field[cur]
+int launch_bulk(Group *g, double dt);   // in place on field[cur], domains with bulk rates;
+                                        // dt <= 0: each domain's own time step
 int launch_aos_to_soa(Group *g, Domain &d, const double *d_aos, void *d_soa);
+int launch_aos_to_soa_f64(Group *g, Domain &d, const double *d_aos, double *d_soa);  // any precision
 int launch_soa_to_aos(Group *g, Domain &d, const void *d_soa, double *d_aos);
 int sweep_lds_attributes(size_t bytes);
 int sync_descriptors(Group *g);         // upload descriptors/prefixes if stale
@@ -135,6 +143,8 @@ int domain_upload(Group *g, int i, const double *rho);
 int domain_download(Group *g, int i, double *rho);
 int domain_set_agents(Group *g, int i, int n, const int *voxel, const double *temp1, const double *temp2,
                       const uint8_t *active);
+int domain_set_bulk(Group *g, int i, const double *supply, const double *target, const double *uptake);
+int group_bulk(Group *g, double dt);
 int group_steps(Group *g, int nsteps, bool with_agents, bool use_graph);
 int group_set_stream(Group *g, void *stream);
 int group_launch_phase(Group *g, int phase, int count);
@@ -144,6 +154,9 @@ int group_autotune(Group *g, int reps, int *chosen);
 int adjoint_forward(Group *g, int nsteps, int every);
 int adjoint_backward(Group *g, const double *seed, double *grad_rho0, double *grad_t1, double *grad_t2);
 void adjoint_free(Group *g);
+// MAT v4 files of a single-domain group (matlab.hip).
+int domain_write_matlab(Group *g, const char *path, const double *origin);
+int domain_read_matlab(Group *g, const char *path, const double *origin);
 
 }  // namespace bfvmb
 

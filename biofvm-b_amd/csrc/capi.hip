@@ -139,7 +139,23 @@ int bfvmb_voxel_indices(int nx, int ny, int nz, double dx, const double *bbox_mi
     return BFVMB_OK;
 }
 
-int bfvmb_simulate_cell_sources_and_sinks(bfvmb_ctx *ctx) { return ctx ? launch_agents(ctx) : BFVMB_ERR_ARG; }
+int bfvmb_simulate_cell_sources_and_sinks(bfvmb_ctx *ctx)
+{
+    if (!ctx)
+        return BFVMB_ERR_ARG;
+    ctx->epoch++;
+    return launch_agents(ctx);
+}
+
+int bfvmb_set_bulk_rates(bfvmb_ctx *ctx, const double *supply, const double *target, const double *uptake)
+{
+    return ctx ? domain_set_bulk(ctx, 0, supply, target, uptake) : BFVMB_ERR_ARG;
+}
+
+int bfvmb_simulate_bulk_sources_and_sinks(bfvmb_ctx *ctx, double dt)
+{
+    return ctx ? group_bulk(ctx, dt) : BFVMB_ERR_ARG;
+}
 
 int bfvmb_simulate_diffusion_decay(bfvmb_ctx *ctx, int nsteps)
 {
@@ -234,6 +250,12 @@ int bfvmb_batch_set_agents(bfvmb_batch *b, int i, int n, const int *voxel, const
                            const uint8_t *active)
 {
     return b ? domain_set_agents(b, i, n, voxel, temp1, temp2, active) : BFVMB_ERR_ARG;
+}
+
+int bfvmb_batch_set_bulk_rates(bfvmb_batch *b, int i, const double *supply, const double *target,
+                               const double *uptake)
+{
+    return b ? domain_set_bulk(b, i, supply, target, uptake) : BFVMB_ERR_ARG;
 }
 
 int bfvmb_batch_run(bfvmb_batch *b, int nsteps, int use_graph)

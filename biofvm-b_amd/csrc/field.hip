@@ -47,6 +47,28 @@ __global__ __launch_bounds__(256) void k_agents(const DomainDesc<T> *__restrict_
     *p = (T)r;
 }
 
+// Bulk sources and sinks (Microenvironment::simulate_bulk_sources_and_sinks,
+// BioFVM_microenvironment.cpp:1110-1153), one thread per (domain, substrate, voxel):
+//   rho = fma(dt, T*S, rho) / (((U + S)*dt) + 1)
+// with T*S and U+S precomputed; the reference's compiled loop fuses only the numerator's
+// multiply-add.  FP64 arithmetic on both storage paths.
+template <class T>
+__global__ __launch_bounds__(256) void k_bulk(const DomainDesc<T> *__restrict__ descs,
+                                              const long long *__restrict__ prefix, int ndom, int parity, double dt)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= prefix[ndom])
+        return;
+    const int di = find_domain_of(prefix, ndom, i);
+    const DomainDesc<T> &D = descs[di];
+    const long long e = i - prefix[di];
+    const double h = dt > 0.0 ? dt : D.dt;
+    T *p = D.field[parity] + e;
+    const double num = fma(h, D.bulk_ts[e], (double)*p);
+    const double den = D.bulk_us[e] * h + 1.0;
+    *p = (T)(num / den);
+}
+
 // rho_aos[((x*ny + y)*nz + z)*nsub + d]  ->  soa[d][z][y][x], via 32x32 (x,z) LDS tiles.
 template <class T>
 __global__ __launch_bounds__(256) void k_aos_to_soa(const double *__restrict__ aos, T *__restrict__ soa,
@@ -124,6 +146,33 @@ int launch_agents(Group *g)
     if (g->precision == BFVMB_FP64)
         return launch_agents_t<double>(g);
     return launch_agents_t<float>(g);
+}
+
+int launch_aos_to_soa_f64(Group *g, Domain &d, const double *d_aos, double *d_soa)
+{
+    hipLaunchKernelGGL(k_aos_to_soa<double>, transpose_grid(d), dim3(32, 8), 0, g->stream, d_aos, d_soa, d.nx, d.ny,
+                       d.nz, d.nsub);
+    return hip_check(g, hipGetLastError(), "aos->soa launch");
+}
+
+int launch_bulk(Group *g, double dt)
+{
+    int rc = sync_descriptors(g);
+    if (rc)
+        return rc;
+    const int ndom = (int)g->doms.size();
+    const long long work = g->prefix[4][ndom];
+    if (work == 0)
+        return BFVMB_OK;
+    const unsigned grid = (unsigned)((work + 255) / 256);
+    const long long *pre = g->d_prefix + (size_t)4 * (ndom + 1);
+    if (g->precision == BFVMB_FP64)
+        hipLaunchKernelGGL(k_bulk<double>, dim3(grid), dim3(256), 0, g->stream,
+                           static_cast<const DomainDesc<double> *>(g->d_desc), pre, ndom, g->cur, dt);
+    else
+        hipLaunchKernelGGL(k_bulk<float>, dim3(grid), dim3(256), 0, g->stream,
+                           static_cast<const DomainDesc<float> *>(g->d_desc), pre, ndom, g->cur, dt);
+    return hip_check(g, hipGetLastError(), "bulk launch");
 }
 
 int launch_aos_to_soa(Group *g, Domain &d, const double *d_aos, void *d_soa)

@@ -108,6 +108,9 @@ std::vector<DomainDesc<T>> build_descs(Group *g)
         e.seg_off = d.seg_off;
         e.t1 = d.t1;
         e.t2 = d.t2;
+        e.bulk_ts = d.bulk;
+        e.bulk_us = d.bulk ? d.bulk + d.nvox * d.nsub : nullptr;
+        e.dt = d.dt;
         e.nseg = d.nseg;
         e.nact = d.nact;
         e.nx = d.nx;
@@ -125,7 +128,7 @@ int sync_descriptors(Group *g)
     if (!g->desc_dirty)
         return BFVMB_OK;
     const size_t nd = g->doms.size();
-    for (int p = 0; p < 4; ++p)
+    for (int p = 0; p < 5; ++p)
         g->prefix[p].assign(nd + 1, 0);
     for (size_t i = 0; i < nd; ++i) {
         const Domain &d = g->doms[i];
@@ -136,9 +139,10 @@ int sync_descriptors(Group *g)
         g->prefix[1][i + 1] = g->prefix[1][i] + tx;
         g->prefix[2][i + 1] = g->prefix[2][i] + ty;
         g->prefix[3][i + 1] = g->prefix[3][i] + tz;
+        g->prefix[4][i + 1] = g->prefix[4][i] + (d.bulk ? (long long)d.nvox * d.nsub : 0);
     }
     std::vector<long long> flat;
-    for (int p = 0; p < 4; ++p)
+    for (int p = 0; p < 5; ++p)
         flat.insert(flat.end(), g->prefix[p].begin(), g->prefix[p].end());
     int rc;
     const size_t desc_bytes = nd * (g->precision == BFVMB_FP64 ? sizeof(DomainDesc<double>) : sizeof(DomainDesc<float>));
@@ -227,6 +231,7 @@ void group_free(Group *g)
     clear_graphs(g);
     for (Domain &d : g->doms) {
         free_agents(d);
+        (void)hipFree(d.bulk);
         (void)hipFree(d.field[0]);
         (void)hipFree(d.field[1]);
         (void)hipFree(d.coef);
@@ -294,6 +299,7 @@ int domain_set_coefficients(Group *g, int i, const double *D, const double *deca
     d.decay.assign(decay, decay + ns);
     d.dt = dt;
     d.have_coef = true;
+    g->desc_dirty = true;  // the descriptor carries dt
     g->epoch++;
     return BFVMB_OK;
 }
@@ -415,10 +421,60 @@ int domain_set_agents(Group *g, int i, int n, const int *voxel, const double *te
     return BFVMB_OK;
 }
 
+int domain_set_bulk(Group *g, int i, const double *supply, const double *target, const double *uptake)
+{
+    if (bad_index(g, i))
+        return set_error(g, BFVMB_ERR_ARG, "set_bulk_rates: bad domain index");
+    Domain &d = g->doms[i];
+    int rc = hip_check(g, hipStreamSynchronize(g->stream), "bulk sync");
+    if (rc)
+        return rc;
+    g->desc_dirty = true;
+    g->epoch++;
+    if (!supply && !target && !uptake) {  // back to the zero_function defaults: no bulk phase
+        (void)hipFree(d.bulk);
+        d.bulk = nullptr;
+        return BFVMB_OK;
+    }
+    // T*S and U+S per element, as the reference's loop rounds them (missing rates are 0)
+    const size_t n = d.nvox * d.nsub;
+    std::vector<double> ts(n), us(n);
+    for (size_t k = 0; k < n; ++k) {
+        const double S = supply ? supply[k] : 0.0, T = target ? target[k] : 0.0, U = uptake ? uptake[k] : 0.0;
+        ts[k] = T * S;
+        us[k] = U + S;
+    }
+    if (!d.bulk && (rc = hip_check(g, hipMalloc((void **)&d.bulk, 2 * n * sizeof(double)), "bulk rates")))
+        return rc;
+    if ((rc = ensure_stage(g, n * sizeof(double))))
+        return rc;
+    const std::vector<double> *src[2] = {&ts, &us};
+    for (int k = 0; k < 2; ++k)
+        if ((rc = hip_check(g, hipMemcpyAsync(g->stage, src[k]->data(), n * sizeof(double), hipMemcpyHostToDevice,
+                                              g->stream), "bulk upload")) ||
+            (rc = launch_aos_to_soa_f64(g, d, static_cast<const double *>(g->stage), d.bulk + k * n)) ||
+            (rc = hip_check(g, hipStreamSynchronize(g->stream), "bulk upload")))
+            return rc;
+    return BFVMB_OK;
+}
+
+int group_bulk(Group *g, double dt)
+{
+    if (!(dt > 0.0))
+        return set_error(g, BFVMB_ERR_ARG, "simulate_bulk_sources_and_sinks: dt must be positive");
+    g->epoch++;
+    return launch_bulk(g, dt);
+}
+
 static int eager_steps(Group *g, int nsteps, bool with_agents)
 {
+    const bool bulk = g->prefix[4].back() > 0;
     for (int s = 0; s < nsteps; ++s) {
         int rc;
+        // time loop of a step: bulk sources/sinks, cell sources/sinks, LOD
+        // (BioFVM_microenvironment.cpp:1110-1169, BioFVM_parallel.cpp:693-1395)
+        if (with_agents && bulk && (rc = launch_bulk(g, 0.0)))
+            return rc;
         if (with_agents && (rc = launch_agents(g)))
             return rc;
         for (int axis = 0; axis < 3; ++axis)

@@ -9,7 +9,7 @@
 // set_internal_uptake_constants, then per step simulate_cell_sources_and_sinks(dt)
 // followed by simulate_diffusion_decay(dt).
 //
-// usage: bfvmb_driver <case.bin> <out.bin> [fp64|fp32] [--fused] [--time]
+// usage: bfvmb_driver <case.bin> <out.bin> [fp64|fp32] [--fused] [--time] [--mat out.mat]
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -40,13 +40,16 @@ T one(FILE *f)
 int main(int argc, char **argv)
 {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: %s case.bin out.bin [fp64|fp32] [--fused] [--time]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s case.bin out.bin [fp64|fp32] [--fused] [--time] [--mat out.mat]\n", argv[0]);
         return 2;
     }
     int precision = BFVMB_FP64;
     bool fused = false, timing = false;
+    const char *mat_path = nullptr;  // --mat PATH: write_to_matlab after the last step
     for (int i = 3; i < argc; ++i) {
-        if (!std::strcmp(argv[i], "fp32"))
+        if (!std::strcmp(argv[i], "--mat") && i + 1 < argc)
+            mat_path = argv[++i];
+        else if (!std::strcmp(argv[i], "fp32"))
             precision = BFVMB_FP32;
         else if (!std::strcmp(argv[i], "--fused"))
             fused = true;
@@ -167,6 +170,8 @@ int main(int argc, char **argv)
             }
         }
         std::fclose(out);
+        if (mat_path)
+            M.write_to_matlab(mat_path);
         if (timing)
             std::printf("{\"steps\": %d, \"ms_total\": %.6f}\n", nsteps, ms);
         biofvmb::delete_all_basic_agents();

@@ -256,6 +256,20 @@ void bfvmb_plugin_invalidate(Microenvironment &M)
         it->second.device_current = false;
 }
 
+void bfvmb_plugin_write_to_matlab(Microenvironment &M, const std::string &filename)
+{
+    // a field newer on the device is written from there (same bytes as
+    // Microenvironment::write_to_matlab, BioFVM_microenvironment.cpp:1079-1104)
+    auto it = g_state.find(&M);
+    if (it != g_state.end() && it->second.ctx && !it->second.host_current && it->second.device_current) {
+        const double lo[3] = {M.mesh.bounding_box[0], M.mesh.bounding_box[1], M.mesh.bounding_box[2]};
+        if (bfvmb_write_matlab(it->second.ctx, filename.c_str(), lo) != BFVMB_OK)
+            fail("bfvmb_write_matlab", it->second.ctx);
+        return;
+    }
+    M.write_to_matlab(filename);
+}
+
 void bfvmb_plugin_release(Microenvironment &M)
 {
     auto it = g_state.find(&M);

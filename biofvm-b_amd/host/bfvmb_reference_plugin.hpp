@@ -14,6 +14,7 @@
 // copy up to date and bfvmb_plugin_invalidate(M) uploads host-side edits on the next call.
 #pragma once
 
+#include <string>
 #include <vector>
 
 void bfvmb_hip_LOD_3D(BioFVM::Microenvironment &M, double dt, int mpi_Size, int mpi_Rank, int *mpi_Coords,
@@ -25,3 +26,5 @@ void bfvmb_plugin_set_resident(BioFVM::Microenvironment &M, bool resident);
 void bfvmb_plugin_sync(BioFVM::Microenvironment &M);        // device -> p_density_vectors
 void bfvmb_plugin_invalidate(BioFVM::Microenvironment &M);  // host field edited: upload next call
 void bfvmb_plugin_release(BioFVM::Microenvironment &M);     // free the device context
+// Microenvironment::write_to_matlab, from the device when it holds the newer field
+void bfvmb_plugin_write_to_matlab(BioFVM::Microenvironment &M, const std::string &filename);

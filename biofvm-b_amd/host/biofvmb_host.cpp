@@ -239,6 +239,21 @@ void Microenvironment::sync_to_host()
     }
 }
 
+void Microenvironment::write_to_matlab(const std::string &filename)
+{
+    sync_to_device();
+    const double lo[3] = {x0_, y0_, z0_};
+    check(bfvmb_write_matlab(ctx_, filename.c_str(), lo), "bfvmb_write_matlab");
+}
+
+void Microenvironment::read_from_matlab(const std::string &filename)
+{
+    sync_to_device();  // host edits first, so that a refused file leaves a consistent field
+    const double lo[3] = {x0_, y0_, z0_};
+    check(bfvmb_read_matlab(ctx_, filename.c_str(), lo), "bfvmb_read_matlab");
+    host_stale_ = true;
+}
+
 double *Microenvironment::density_data()
 {
     sync_to_host();

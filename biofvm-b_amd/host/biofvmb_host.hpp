@@ -110,6 +110,14 @@ public:
     void sync_to_host();
     void sync_to_device();
 
+    // ---- MAT v4 files of the field
+    // Microenvironment::write_to_matlab (BioFVM_microenvironment.cpp:1079-1104): written
+    // from the device, where the field lives, after any pending host edits are uploaded
+    void write_to_matlab(const std::string &filename);
+    // loads a multiscale_microenvironment file written for this mesh into the field (the
+    // reference declares read_from_matlab, BioFVM_microenvironment.h:284, without a body)
+    void read_from_matlab(const std::string &filename);
+
     // ---- the hot path
     // diffusion_decay_solver slot, LOD_3D: the first call freezes dt, D and decay
     // (BioFVM_parallel.cpp:714-918), as the reference does.

@@ -30,6 +30,7 @@ extern "C" {
 #define BFVMB_ERR_HIP 3     /* HIP runtime failure */
 #define BFVMB_ERR_NOMEM 4   /* device allocation failed */
 #define BFVMB_ERR_NODEV 5   /* no usable gfx950 device */
+#define BFVMB_ERR_IO 6      /* file could not be opened, read or written, or is malformed */
 
 #define BFVMB_FP64 0        /* drop-in path: FP64 storage and sweeps */
 #define BFVMB_FP32 1        /* bulk path: FP32 storage and sweeps, FP64 Thomas set-up */
@@ -90,12 +91,26 @@ int bfvmb_voxel_indices(int nx, int ny, int nz, double dx, const double *bbox_mi
 /* Microenvironment::simulate_cell_sources_and_sinks (BioFVM_microenvironment.cpp:1155-1169). */
 int bfvmb_simulate_cell_sources_and_sinks(bfvmb_ctx *ctx);
 
+/* Bulk sources and sinks: the per-voxel supply rate S, supply target T and uptake rate U
+ * that the reference's bulk_supply_rate_function / bulk_supply_target_densities_function /
+ * bulk_uptake_rate_function slots write (BioFVM_microenvironment.h:165-167), given as
+ * voxel-major [voxel][substrate] arrays (NULL = the zero_function default).  T*S and U+S
+ * are formed once here (update_rates, BioFVM_microenvironment.cpp:1171-1198) and kept on
+ * the device; all three NULL removes the bulk phase.  Once set, each step of bfvmb_run /
+ * bfvmb_run_graph starts with the bulk update (time step of bfvmb_set_coefficients). */
+int bfvmb_set_bulk_rates(bfvmb_ctx *ctx, const double *supply, const double *target, const double *uptake);
+
+/* Microenvironment::simulate_bulk_sources_and_sinks (BioFVM_microenvironment.cpp:1110-1153):
+ * rho = (rho + dt*T*S) / (1 + dt*(U + S)) per voxel and substrate, with the reference's
+ * rounding. */
+int bfvmb_simulate_bulk_sources_and_sinks(bfvmb_ctx *ctx, double dt);
+
 /* nsteps calls of the diffusion_decay_solver slot (BioFVM_parallel.cpp:693-1395): per step
  * Dirichlet, x sweep, Dirichlet, y sweep, Dirichlet, z sweep, Dirichlet. */
 int bfvmb_simulate_diffusion_decay(bfvmb_ctx *ctx, int nsteps);
 
 /* nsteps of the demo time loop (complete_diffusion_demo.cpp:489-493): per step, cell
- * sources/sinks then one LOD step. */
+ * sources/sinks then one LOD step (preceded by the bulk update when bulk rates are set). */
 int bfvmb_run(bfvmb_ctx *ctx, int nsteps);
 
 /* bfvmb_run captured once as a HIP graph (nsteps x 4 kernel nodes) and replayed as one
@@ -141,6 +156,21 @@ int bfvmb_adjoint_forward(bfvmb_ctx *ctx, int nsteps, int checkpoint_every);
 int bfvmb_adjoint_backward(bfvmb_ctx *ctx, const double *seed, double *grad_rho0, double *grad_temp1,
                            double *grad_temp2);
 
+/* ---- MAT v4 output and input of the field.
+ * bfvmb_write_matlab writes the file Microenvironment::write_to_matlab writes
+ * (BioFVM_microenvironment.cpp:1079-1104; MPI-IO variant BioFVM_parallel.cpp:610-678 at
+ * world size 1), byte for byte: variable "multiscale_microenvironment", one column per
+ * voxel in mesh order holding center x, y, z, volume and the densities.  The columns are
+ * assembled on the GPU from the resident field (FP32 fields are widened), so no host copy
+ * of the field is needed.  `bbox_min` is the mesh's lower corner (x, y, z; NULL = origin).
+ * bfvmb_read_matlab loads such a file into the field, which the reference leaves unwritten
+ * (Microenvironment::read_from_matlab, BioFVM_microenvironment.h:284): the matrix must be
+ * (4 + nsub) x voxels of doubles (read_matlab's checks, BioFVM_matlab.cpp:71-200) and every
+ * column's center and volume must lie on the context's grid, else the field is unchanged
+ * and BFVMB_ERR_ARG / BFVMB_ERR_IO is returned. */
+int bfvmb_write_matlab(bfvmb_ctx *ctx, const char *filename, const double *bbox_min);
+int bfvmb_read_matlab(bfvmb_ctx *ctx, const char *filename, const double *bbox_min);
+
 /* LDS bytes per workgroup and CU count of HIP device `device`. */
 int bfvmb_device_info(int device, int *lds_per_block, int *cu_count);
 
@@ -166,8 +196,11 @@ int bfvmb_batch_upload_density(bfvmb_batch *b, int i, const double *rho_voxel_ma
 int bfvmb_batch_download_density(bfvmb_batch *b, int i, double *rho_voxel_major);
 int bfvmb_batch_set_agents(bfvmb_batch *b, int i, int n, const int *voxel, const double *temp1,
                            const double *temp2, const uint8_t *active);
-/* nsteps of (sources/sinks, LOD step) for every member; use_graph != 0 replays a captured
- * graph of the whole run. */
+/* bfvmb_set_bulk_rates for member i. */
+int bfvmb_batch_set_bulk_rates(bfvmb_batch *b, int i, const double *supply, const double *target,
+                               const double *uptake);
+/* nsteps of (bulk sources/sinks where set, cell sources/sinks, LOD step) for every member;
+ * use_graph != 0 replays a captured graph of the whole run. */
 int bfvmb_batch_run(bfvmb_batch *b, int nsteps, int use_graph);
 int bfvmb_batch_sync(bfvmb_batch *b);
 int bfvmb_batch_set_launch(bfvmb_batch *b, int sweep, int grid_blocks, int lds_blocks_per_cu,

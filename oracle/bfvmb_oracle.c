@@ -211,6 +211,25 @@ ORC_API void orc_sources_sinks(int nagents, int nsub, const int *voxel,
     }
 }
 
+/* Bulk sources and sinks, Microenvironment::simulate_bulk_sources_and_sinks
+ * (BioFVM_microenvironment.cpp:1110-1153), with supply S, target T and uptake U per voxel
+ * and substrate ([nvox][nsub], what the bulk_*_function slots write):
+ *   temp2 = T*S;  rho += dt*temp2  (one fused multiply-add);
+ *   temp3 = U + S;  temp3 *= dt;  temp3 += 1;  rho /= temp3  (each rounded on its own,
+ *   the statements sit in separate loops).                                               */
+ORC_API void orc_bulk_sources_sinks(size_t nvox, int nsub, double dt, const double *S, const double *T,
+                                    const double *U, double *rho)
+{
+    for (size_t i = 0; i < nvox * (size_t)nsub; ++i) {
+        const double ts = T[i] * S[i];
+        const double num = fma(dt, ts, rho[i]);
+        double den = U[i] + S[i];
+        den = den * dt;
+        den = den + 1.0;
+        rho[i] = num / den;
+    }
+}
+
 /* Agent -> voxel, Basic_Agent::assign_position + update_voxel_index
  * (BioFVM_basic_agent.cpp:112-153) with Cartesian_Mesh::nearest_voxel_local_index at world
  * size 1 (BioFVM_parallel.cpp:418-522) and is_position_valid (BioFVM_mesh.cpp:204-213).

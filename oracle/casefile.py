@@ -42,6 +42,7 @@ class Case:
     snaps: list
     with_agents: bool = True
     x0: float = 0.0
+    bulk: object = None  # (supply, target, uptake), each [nx, ny, nz, nsub]; bulk update per step
 
     @property
     def nagents(self) -> int:
@@ -97,16 +98,29 @@ def read_output(path: str, case: Case) -> dict:
 
 
 def run_reference(case: Case, workdir: str, solver: str = "lod", granularity: int = 1,
-                  threads: int = 1, timing: bool = False) -> dict:
-    """Run the compiled reference on ``case``; returns its agent mapping and snapshots."""
+                  threads: int = 1, timing: bool = False, mat: bool = False) -> dict:
+    """Run the compiled reference on ``case``; returns its agent mapping and snapshots, and
+    with ``mat`` the bytes of Microenvironment::write_to_matlab after the last step."""
     cpath = os.path.join(workdir, "case.bin")
     opath = os.path.join(workdir, "out.bin")
+    mpath = os.path.join(workdir, "out.mat")
     case.write(cpath)
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     args = [REF_DRIVER, cpath, opath, solver, str(granularity)] + (["--time"] if timing else [])
+    if mat:
+        args += ["--mat", mpath]
+    if case.bulk is not None:
+        bpath = os.path.join(workdir, "bulk.bin")
+        with open(bpath, "wb") as f:
+            for a in case.bulk:
+                np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+        args += ["--bulk", bpath]
     proc = subprocess.run(args, env=env, capture_output=True, text=True, check=True)
     out = read_output(opath, case)
     out["stdout"] = proc.stdout
+    if mat:
+        with open(mpath, "rb") as f:
+            out["mat"] = f.read()
     return out
 
 

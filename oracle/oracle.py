@@ -41,6 +41,7 @@ def lib():
         L.orc_voxel_index.argtypes = [_dp, _dp, d, d, d, i, i, i]
         L.orc_voxel_index.restype = i
         L.orc_run.argtypes = [i, i, i, i, d, d, _dp, _dp, _dp, _bp, i, _ip, _bp, _dp, _dp, i, _dp, _dp]
+        L.orc_bulk_sources_sinks.argtypes = [ctypes.c_size_t, i, d, _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -155,9 +156,24 @@ def run(case, nsteps=None, rho=None, with_agents=None, tables=None):
     k1, p1 = _d(t1 if na else np.zeros(1))
     k2, p2 = _d(t2 if na else np.zeros(1))
     work = np.empty((3 * (case.nx + case.ny + case.nz) + 4) * case.nsub)
-    lib().orc_run(case.nx, case.ny, case.nz, case.nsub, case.dx, case.dt, pD, pdc, pdv, pda, na, pv, pa,
-                  p1, p2, nsteps, rho.ctypes.data_as(_dp), work.ctypes.data_as(_dp))
+    if case.bulk is None:
+        lib().orc_run(case.nx, case.ny, case.nz, case.nsub, case.dx, case.dt, pD, pdc, pdv, pda, na, pv, pa,
+                      p1, p2, nsteps, rho.ctypes.data_as(_dp), work.ctypes.data_as(_dp))
+        return rho
+    # per step: bulk sources/sinks, then the cell update and the LOD step
+    for _ in range(nsteps):
+        bulk_sources_sinks(rho, case.dt, *case.bulk)
+        lib().orc_run(case.nx, case.ny, case.nz, case.nsub, case.dx, case.dt, pD, pdc, pdv, pda, na, pv, pa,
+                      p1, p2, 1, rho.ctypes.data_as(_dp), work.ctypes.data_as(_dp))
     return rho
+
+
+def bulk_sources_sinks(rho, dt, supply, target, uptake):
+    """In place on the AoS field (orc_bulk_sources_sinks)."""
+    keep = [np.ascontiguousarray(a, dtype=np.float64).reshape(rho.shape) for a in (supply, target, uptake)]
+    nvox = rho.size // rho.shape[-1]
+    lib().orc_bulk_sources_sinks(nvox, rho.shape[-1], dt,
+                                 *[k.ctypes.data_as(_dp) for k in keep], rho.ctypes.data_as(_dp))
 
 
 def run_snapshots(case):

@@ -12,7 +12,8 @@
 //   set_total_volume / rates / set_internal_uptake_constants, then per step
 //   simulate_cell_sources_and_sinks(dt) followed by simulate_diffusion_decay(...).
 //
-// usage: ref_driver <case.bin> <out.bin> [lod|avx] [granularity] [--time]
+// usage: ref_driver <case.bin> <out.bin> [lod|avx] [granularity] [--time] [--mat out.mat]
+//                   [--bulk rates.bin]
 //        ref_driver_hip <case.bin> <out.bin> [hip|hipres] ...   (BFVMB_PLUGIN build: the
 //        GPU solver in the diffusion_decay_solver slot; hipres keeps the field resident)
 #include <mpi.h>
@@ -53,6 +54,26 @@ struct Reader {
     template <class T> T one() { T v; get(&v, 1); return v; }
 };
 
+// bulk supply / target / uptake per voxel ([nvox][nsub] each), served to the reference
+// through its bulk_*_function slots (BioFVM_microenvironment.h:165-167)
+std::vector<double> g_bulk[3];
+int g_bulk_nsub = 0;
+void bulk_supply(Microenvironment *, int v, double *dst)
+{
+    for (int p = 0; p < g_bulk_nsub; ++p)
+        dst[p] = g_bulk[0][(size_t)v * g_bulk_nsub + p];
+}
+void bulk_target(Microenvironment *, int v, double *dst)
+{
+    for (int p = 0; p < g_bulk_nsub; ++p)
+        dst[p] = g_bulk[1][(size_t)v * g_bulk_nsub + p];
+}
+void bulk_uptake(Microenvironment *, int v, double *dst)
+{
+    for (int p = 0; p < g_bulk_nsub; ++p)
+        dst[p] = g_bulk[2][(size_t)v * g_bulk_nsub + p];
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
@@ -64,6 +85,15 @@ int main(int argc, char **argv)
     std::string solver = argc > 3 ? argv[3] : "lod";
     int gran = argc > 4 ? atoi(argv[4]) : 1;
     bool timing = argc > 5 && std::string(argv[5]) == "--time";
+    std::string mat_path;  // --mat PATH: Microenvironment::write_to_matlab after the last step
+    std::string bulk_path;  // --bulk PATH: supply, target, uptake [nvox][nsub] doubles; per step
+                            // simulate_bulk_sources_and_sinks runs before the cell update
+    for (int i = 3; i + 1 < argc; ++i) {
+        if (std::string(argv[i]) == "--mat")
+            mat_path = argv[i + 1];
+        if (std::string(argv[i]) == "--bulk")
+            bulk_path = argv[i + 1];
+    }
 
     int provided = 0;
     MPI_Init_thread(nullptr, nullptr, MPI_THREAD_FUNNELED, &provided);
@@ -128,6 +158,19 @@ int main(int argc, char **argv)
     }
 #endif
     M.granurality = gran;
+    if (!bulk_path.empty()) {
+        Reader b{fopen(bulk_path.c_str(), "rb")};
+        if (!b.f) { perror(bulk_path.c_str()); return 2; }
+        g_bulk_nsub = nsub;
+        for (auto &v : g_bulk) {
+            v.resize(nvox * nsub);
+            b.get(v.data(), v.size());
+        }
+        fclose(b.f);
+        M.bulk_supply_rate_function = bulk_supply;
+        M.bulk_supply_target_densities_function = bulk_target;
+        M.bulk_uptake_rate_function = bulk_uptake;
+    }
     std::copy(init.begin(), init.end(), M.p_density_vectors->begin());
 
     std::vector<int> voxel(nagents);
@@ -165,6 +208,17 @@ int main(int argc, char **argv)
     double total_ms = 0.0;
     for (int s = 1; s <= nsteps; ++s) {
         auto t_a = std::chrono::steady_clock::now();
+        if (!bulk_path.empty()) {
+#ifdef BFVMB_PLUGIN
+            if (hip)
+                bfvmb_plugin_sync(M);  // the bulk update runs on the host copy
+#endif
+            M.simulate_bulk_sources_and_sinks(dt);
+#ifdef BFVMB_PLUGIN
+            if (hip)
+                bfvmb_plugin_invalidate(M);
+#endif
+        }
         if (with_agents) {
 #ifdef BFVMB_PLUGIN
             if (hip)
@@ -191,6 +245,14 @@ int main(int argc, char **argv)
         }
     }
     fclose(out);
+    if (!mat_path.empty()) {
+#ifdef BFVMB_PLUGIN
+        if (hip)
+            bfvmb_plugin_write_to_matlab(M, mat_path);  // from the device when resident
+        else
+#endif
+            M.write_to_matlab(mat_path);
+    }
     if (timing)
         printf("{\"steps\": %d, \"ms_total\": %.6f, \"ms_per_step\": %.6f, \"threads\": %d}\n", nsteps,
                total_ms, total_ms / nsteps, omp_get_max_threads());

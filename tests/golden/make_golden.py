@@ -7,7 +7,9 @@ constants temp1/temp2, and full density fields at the snapshot steps.  The refer
 AVX2 solver (granularity 3) is run on the same case and must agree bit for bit with its
 scalar solver; that is recorded in the fixture.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py          # the .npz fixtures
+    python tests/golden/make_golden.py --mat    # MAT v4 fixtures (<name>.mat + <name>.npz)
+    python tests/golden/make_golden.py --bulk   # bulk source/sink fixtures
 """
 import os
 import sys
@@ -40,21 +42,77 @@ def cases():
     yield "long_x_300x17x19x1", cf.make_case(300, 17, 19, 1, seed=7, nagents=100, nsteps=4)
 
 
+def mat_cases():
+    """Cases whose final field the reference also writes as a MAT v4 file
+    (Microenvironment::write_to_matlab), kept as <name>.mat beside the .npz."""
+    c = cf.make_case(13, 9, 11, 3, seed=15, dx=15.0, nagents=60, nsteps=6, snaps=[6], shared_voxels=True,
+                     dactive=[1, 0, 1])
+    c.x0 = -250.0
+    yield "mat_offset_13x9x11x3", c
+    yield "mat_1x1x2x1", cf.make_case(1, 1, 2, 1, seed=16, nagents=2, nsteps=2, snaps=[2])
+
+
+def _bulk(case, seed, sparse):
+    rng = np.random.default_rng(seed)
+    shape = (case.nx, case.ny, case.nz, case.nsub)
+    supply = rng.uniform(0.0, 5.0, shape)
+    if sparse:
+        supply *= rng.uniform(0.0, 1.0, shape) < 0.05  # a few supplying voxels
+    return supply, rng.uniform(0.0, 40.0, shape), rng.uniform(0.0, 20.0, shape)
+
+
+def bulk_cases():
+    """Cases with Microenvironment::simulate_bulk_sources_and_sinks before each step."""
+    c = cf.make_case(15, 12, 10, 3, seed=17, nagents=80, nsteps=5, snaps=[1, 5], shared_voxels=True,
+                     dactive=[1, 0, 0])
+    c.bulk = _bulk(c, 18, sparse=False)
+    yield "bulk_15x12x10x3", c
+    c = cf.make_case(8, 1, 9, 2, seed=19, nagents=0, nsteps=4, snaps=[4], with_agents=False)
+    c.bulk = _bulk(c, 20, sparse=True)
+    yield "bulk_8x1x9x2", c
+
+
+def main_bulk():
+    for name, case in bulk_cases():
+        with tempfile.TemporaryDirectory() as d:
+            out = cf.run_reference(case, d, solver="lod", granularity=1)
+        path = save(name, case, out, avx_equal=True)
+        print(f"{name}: {os.path.getsize(path) / 1e3:.0f} kB")
+
+
+def main_mat():
+    for name, case in mat_cases():
+        with tempfile.TemporaryDirectory() as d:
+            out = cf.run_reference(case, d, solver="lod", granularity=1, mat=True)
+        save(name, case, out, avx_equal=True)
+        with open(os.path.join(HERE, f"{name}.mat"), "wb") as f:
+            f.write(out["mat"])
+        print(f"{name}.mat: {len(out['mat'])} bytes")
+
+
+def save(name, case, out, avx_equal):
+    arrays = dict(nx=case.nx, ny=case.ny, nz=case.nz, nsub=case.nsub, dx=case.dx, dt=case.dt, D=case.D,
+                  decay=case.decay, dvalue=case.dvalue, dactive=case.dactive, init=case.init, pos=case.pos,
+                  volume=case.volume, S=case.S, U=case.U, T=case.T, nsteps=case.nsteps,
+                  snaps=np.asarray(case.snaps), with_agents=int(case.with_agents), voxel=out["voxel"],
+                  temp1=out["temp1"], temp2=out["temp2"], avx_equal=int(avx_equal), x0=case.x0)
+    if case.bulk is not None:
+        for k, a in zip(("bulk_supply", "bulk_target", "bulk_uptake"), case.bulk):
+            arrays[k] = a
+    for s, f in out["fields"].items():
+        arrays[f"field_{s}"] = f
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **arrays)
+    return path
+
+
 def main():
     for name, case in cases():
         with tempfile.TemporaryDirectory() as d:
             out = cf.run_reference(case, d, solver="lod", granularity=1)
             avx = cf.run_reference(case, d, solver="avx", granularity=3)
         avx_equal = all(np.array_equal(out["fields"][s], avx["fields"][s]) for s in case.snaps)
-        arrays = dict(nx=case.nx, ny=case.ny, nz=case.nz, nsub=case.nsub, dx=case.dx, dt=case.dt, D=case.D,
-                      decay=case.decay, dvalue=case.dvalue, dactive=case.dactive, init=case.init, pos=case.pos,
-                      volume=case.volume, S=case.S, U=case.U, T=case.T, nsteps=case.nsteps,
-                      snaps=np.asarray(case.snaps), with_agents=int(case.with_agents), voxel=out["voxel"],
-                      temp1=out["temp1"], temp2=out["temp2"], avx_equal=int(avx_equal))
-        for s, f in out["fields"].items():
-            arrays[f"field_{s}"] = f
-        path = os.path.join(HERE, f"{name}.npz")
-        np.savez_compressed(path, **arrays)
+        path = save(name, case, out, avx_equal)
         print(f"{name}: {os.path.getsize(path) / 1e3:.0f} kB, avx==lod {avx_equal}")
 
 
@@ -63,7 +121,10 @@ def load(name):
     z = np.load(os.path.join(HERE, f"{name}.npz"), allow_pickle=False)
     case = cf.Case(int(z["nx"]), int(z["ny"]), int(z["nz"]), int(z["nsub"]), float(z["dx"]), float(z["dt"]),
                    z["D"], z["decay"], z["dvalue"], z["dactive"], z["init"], z["pos"], z["volume"], z["S"], z["U"],
-                   z["T"], int(z["nsteps"]), [int(s) for s in z["snaps"]], bool(int(z["with_agents"])))
+                   z["T"], int(z["nsteps"]), [int(s) for s in z["snaps"]], bool(int(z["with_agents"])),
+                   float(z["x0"]) if "x0" in z.files else 0.0,
+                   tuple(z[k] for k in ("bulk_supply", "bulk_target", "bulk_uptake")) if "bulk_supply" in z.files
+                   else None)
     ref = {"voxel": z["voxel"], "temp1": z["temp1"], "temp2": z["temp2"], "avx_equal": bool(int(z["avx_equal"])),
            "fields": {int(s): z[f"field_{int(s)}"] for s in z["snaps"]}}
     return case, ref
@@ -72,5 +133,21 @@ def load(name):
 NAMES = ["config1_50cube", "aniso_17x23x31x3", "nx1_1x5x7x2", "ny1_2x1x9x4", "nz1_6x4x1x2", "one_voxel_1x1x1x3",
          "conserve_20x18x16x2", "multi_24x20x28x5", "defaults_12x10x14x2", "long_x_300x17x19x1"]
 
+MAT_NAMES = ["mat_offset_13x9x11x3", "mat_1x1x2x1"]
+BULK_NAMES = ["bulk_15x12x10x3", "bulk_8x1x9x2"]
+ALL_NAMES = NAMES + MAT_NAMES
+
+
+def load_mat(name):
+    """Bytes of the reference's MAT v4 file for fixture ``name``."""
+    with open(os.path.join(HERE, f"{name}.mat"), "rb") as f:
+        return f.read()
+
+
 if __name__ == "__main__":
-    main()
+    if "--mat" in sys.argv[1:]:
+        main_mat()
+    elif "--bulk" in sys.argv[1:]:
+        main_bulk()
+    else:
+        main()

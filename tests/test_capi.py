@@ -67,7 +67,7 @@ def test_voxel_mapping_matches_reference_fixtures():
     sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
     import make_golden as golden
     from biofvmb.solver import voxel_indices
-    for name in golden.NAMES:
+    for name in golden.ALL_NAMES:
         case, ref = golden.load(name)
         got = voxel_indices((case.nx, case.ny, case.nz), case.dx, case.pos, case.bbox[:3])
         assert np.array_equal(np.where(got < 0, 0, got), ref["voxel"]), name

@@ -14,7 +14,7 @@ import make_golden as golden  # noqa: E402
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", golden.NAMES)
+@pytest.mark.parametrize("name", golden.ALL_NAMES)
 def test_fp64_matches_reference_fixture(name):
     from biofvmb import LODSolver
     case, ref = golden.load(name)

@@ -21,7 +21,7 @@ DRIVER = os.path.join(ROOT, "biofvm-b_amd", "lib", "bfvmb_driver")
 
 
 @pytest.mark.parametrize("mode", ["per_call", "fused"])
-@pytest.mark.parametrize("name", golden.NAMES)
+@pytest.mark.parametrize("name", golden.ALL_NAMES)
 def test_host_driver_matches_reference(name, mode):
     case, ref = golden.load(name)
     print(f"[gpu] host driver {name} {mode}", flush=True)
@@ -36,3 +36,19 @@ def test_host_driver_matches_reference(name, mode):
     assert np.array_equal(out["temp2"], ref["temp2"])
     for step, f in ref["fields"].items():
         assert np.array_equal(out["fields"][step], f), f"{name} step {step}: {(out['fields'][step] != f).sum()} differ"
+
+
+@pytest.mark.parametrize("mode", ["per_call", "fused"])
+@pytest.mark.parametrize("name", golden.MAT_NAMES)
+def test_host_driver_writes_reference_mat_file(name, mode):
+    """Microenvironment::write_to_matlab of the host layer (written from the device) equals
+    the file the reference wrote after the same run, byte for byte."""
+    case, ref = golden.load(name)
+    print(f"[gpu] host driver MAT {name} {mode}", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        cpath, opath, mpath = (os.path.join(d, f) for f in ("case.bin", "out.bin", "out.mat"))
+        case.write(cpath)
+        args = [DRIVER, cpath, opath, "fp64", "--mat", mpath] + (["--fused"] if mode == "fused" else [])
+        subprocess.run(args, check=True, timeout=120)
+        with open(mpath, "rb") as f:
+            assert f.read() == golden.load_mat(name)

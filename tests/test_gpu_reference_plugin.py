@@ -24,7 +24,7 @@ DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver_hip")
 
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="reference build with the plugin (make -C oracle ref-hip) absent")
 @pytest.mark.parametrize("mode", ["hip", "hipres"])
-@pytest.mark.parametrize("name", golden.NAMES)
+@pytest.mark.parametrize("name", golden.ALL_NAMES)
 def test_reference_driver_with_gpu_slot(name, mode):
     case, ref = golden.load(name)
     print(f"[gpu] reference driver + plugin {name} {mode}", flush=True)
@@ -37,3 +37,20 @@ def test_reference_driver_with_gpu_slot(name, mode):
     assert np.array_equal(out["voxel"], ref["voxel"])
     for step, f in ref["fields"].items():
         assert np.array_equal(out["fields"][step], f), f"{name} step {step}: {(out['fields'][step] != f).sum()} differ"
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="reference build with the plugin (make -C oracle ref-hip) absent")
+@pytest.mark.parametrize("mode", ["hip", "hipres"])
+@pytest.mark.parametrize("name", golden.MAT_NAMES)
+def test_reference_driver_mat_file(name, mode):
+    """bfvmb_plugin_write_to_matlab in the reference's own driver (from the device when the
+    field is resident) writes the reference's MAT file byte for byte."""
+    case, ref = golden.load(name)
+    print(f"[gpu] reference driver + plugin MAT {name} {mode}", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        cpath, opath, mpath = (os.path.join(d, f) for f in ("case.bin", "out.bin", "out.mat"))
+        case.write(cpath)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([DRIVER, cpath, opath, mode, "1", "--mat", mpath], check=True, timeout=120, env=env)
+        with open(mpath, "rb") as f:
+            assert f.read() == golden.load_mat(name)

@@ -20,7 +20,7 @@ import make_golden as golden  # noqa: E402
 
 
 # ---------------------------------------------------------------- 1. reference fixtures
-@pytest.mark.parametrize("name", golden.NAMES)
+@pytest.mark.parametrize("name", golden.ALL_NAMES)
 def test_oracle_matches_reference_bitwise(name):
     case, ref = golden.load(name)
     voxel, t1, t2 = orc.agent_tables(case)
@@ -32,7 +32,7 @@ def test_oracle_matches_reference_bitwise(name):
         assert np.array_equal(got[s], f), f"{name} step {s}: {(got[s] != f).sum()} values differ"
 
 
-@pytest.mark.parametrize("name", golden.NAMES)
+@pytest.mark.parametrize("name", golden.ALL_NAMES)
 def test_reference_avx_solver_equals_scalar_solver(name):
     _, ref = golden.load(name)
     assert ref["avx_equal"]
@@ -170,3 +170,35 @@ def test_voxel_mapping_brute_force():
     c = np.clip(np.floor((case.pos - lo) / case.dx).astype(int), 0, n - 1)
     want = np.where(inside, (c[:, 0] * n[1] + c[:, 1]) * n[2] + c[:, 2], 0)
     assert np.array_equal(got, want)
+
+
+# ---------------------------------------------------------------- bulk sources and sinks
+@pytest.mark.parametrize("name", golden.BULK_NAMES)
+def test_bulk_matches_reference_fixtures(name):
+    """orc_bulk_sources_sinks inside the time loop, bit for bit against the reference's
+    Microenvironment::simulate_bulk_sources_and_sinks run by its own driver."""
+    case, ref = golden.load(name)
+    for step, field in ref["fields"].items():
+        assert np.array_equal(orc.run(case, nsteps=step), field), f"{name} step {step}"
+
+
+def test_bulk_closed_forms():
+    rng = np.random.default_rng(2)
+    rho0 = rng.lognormal(0.0, 0.5, (4, 3, 5, 2))
+    z = np.zeros_like(rho0)
+    r = rho0.copy()
+    orc.bulk_sources_sinks(r, 0.01, z, z, z)  # no rates: unchanged
+    assert np.array_equal(r, rho0)
+    U = rng.uniform(0.0, 20.0, rho0.shape)
+    r = rho0.copy()
+    orc.bulk_sources_sinks(r, 0.01, z, z, U)  # uptake only: one implicit decay step
+    assert np.array_equal(r, rho0 / (U * 0.01 + 1.0))
+    S = rng.uniform(0.1, 5.0, rho0.shape)
+    T = rng.uniform(1.0, 40.0, rho0.shape)
+    r = T.copy()
+    orc.bulk_sources_sinks(r, 0.01, S, T, z)  # at the target with no uptake: a fixed point
+    assert np.allclose(r, T, rtol=4e-16, atol=0)
+    r = rho0.copy()  # long run with supply only converges to the target
+    for _ in range(4000):
+        orc.bulk_sources_sinks(r, 0.5, S, T, z)
+    assert np.allclose(r, T, rtol=1e-12, atol=0)
