@@ -95,6 +95,94 @@ def time_phases(s, torch, count=10):
     return out
 
 
+def bench_batched(torch, stream, steps=20, seed=3):
+    """64 independent microenvironments of uneven sizes: nx, ny, nz ~ U{32..256}, nsub ~
+    U{1..8}, D ~ logU[1e2, 1e5], decay ~ logU[1e-3, 1], U{0..50000} agents each; the
+    autotuner picks the launch shapes, then `steps` steps run as one captured graph, in FP64
+    and in FP32 storage."""
+    from biofvmb import Batch
+    rng = np.random.default_rng(seed)
+    m = 64
+    shapes = [tuple(int(v) for v in rng.integers(32, 257, 3)) + (int(rng.integers(1, 9)),) for _ in range(m)]
+    params = []
+    for (nx, ny, nz, ns) in shapes:
+        Dm = 10.0 ** rng.uniform(2, 5, ns)
+        lam = 10.0 ** rng.uniform(-3, 0, ns)
+        na = int(rng.integers(0, 50001))
+        ext = np.array([nx, ny, nz]) * DX
+        pos = rng.uniform(0, 1, (na, 3)) * ext
+        params.append((Dm, lam, pos, rng.uniform(0, 5, (na, ns)), rng.uniform(0, 20, (na, ns))))
+    total_vs = sum(nx * ny * nz * ns for nx, ny, nz, ns in shapes)
+    out = {"members": m, "voxel_substrates": total_vs, "steps": steps}
+    for prec in ("fp64", "fp32"):
+        b = Batch(shapes, DX, prec)
+        b.set_stream(stream)
+        for i, ((nx, ny, nz, ns), (Dm, lam, pos, S, U)) in enumerate(zip(shapes, params)):
+            b.set_coefficients(i, Dm, lam, DT)
+            b.set_dirichlet(i, np.full(ns, 10.0), np.arange(ns) % 2 == 0)
+            b.upload(i, np.full((nx, ny, nz, ns), 1.0 + 0.01 * i))
+            from biofvmb.solver import voxel_indices
+            vox = voxel_indices((nx, ny, nz), DX, pos)
+            vox[vox < 0] = 0
+            vol = 4.0 / 3.0 * np.pi * 125.0
+            k = (DT * vol) / (DX * DX * DX)
+            t1 = (S * 1.0) * k
+            t2 = 1.0 + k * U + k * S
+            b.set_agents(i, vox, t1, t2)
+        chosen = b.autotune(reps=2)
+        b.run(steps, graph=True)  # capture + warm-up
+        torch.cuda.synchronize()
+        a, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        b.run(steps, graph=True)
+        e.record()
+        torch.cuda.synchronize()
+        sec = a.elapsed_time(e) / 1e3
+        b.close()
+        out[prec] = {"value": total_vs * steps / sec, "ms_per_step": sec * 1e3 / steps,
+                     "autotune": np.asarray(chosen).tolist()}
+        log(f"batched {prec}: {total_vs * steps / sec:.4e} vs/s, {sec * 1e3 / steps:.3f} ms/step")
+    return out
+
+
+def bench_adjoint(torch, stream, n=128, nsteps=50, every=10, nagents=10_000, seed=5):
+    """Gradient of sum((rho_T - target)^2) after 50 steps at 128^3 x 2 with 10k agents:
+    forward run with checkpoints every 10 steps, then the reverse pass."""
+    from biofvmb import LODSolver
+    rng = np.random.default_rng(seed)
+    ns = 2
+    s = LODSolver(n, n, n, ns, DX, "fp64")
+    s.set_stream(stream)
+    s.set_coefficients([1e5, 1e3], [0.1, 0.01], DT)
+    s.set_dirichlet([0.0, 0.0], [0, 0])
+    init = rng.lognormal(0.0, 0.5, (n, n, n, ns))
+    s.upload(init)
+    pos = rng.uniform(0, n * DX, (nagents, 3))
+    vox = s.voxel_indices(pos)
+    s.set_agent_rates(vox, np.full(nagents, 4.0 / 3.0 * np.pi * 125.0), rng.uniform(0, 5, (nagents, ns)),
+                      rng.uniform(0, 20, (nagents, ns)), np.ones((nagents, ns)), DT)
+    target = rng.lognormal(0.0, 0.5, (n, n, n, ns))
+    s.adjoint_forward(nsteps, every)  # warm-up (captures the segment graphs)
+    s.adjoint_backward(target)
+    s.upload(init)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.adjoint_forward(nsteps, every)
+    s.sync()
+    t1 = time.perf_counter()
+    rT = s.download()
+    t2 = time.perf_counter()
+    s.adjoint_backward(2.0 * (rT - target))
+    t3 = time.perf_counter()
+    s.close()
+    vs = n ** 3 * ns * nsteps
+    out = {"workload": f"{n}^3x{ns}, {nagents} agents, {nsteps} steps, checkpoint every {every}",
+           "forward_s": t1 - t0, "backward_s": t3 - t2, "value": vs / ((t1 - t0) + (t3 - t2)),
+           "unit": "voxel-substrate updates/s (forward + reverse pass, host transfers included)"}
+    log(f"adjoint: forward {t1 - t0:.3f} s, backward {t3 - t2:.3f} s, {out['value']:.3e} vs/s")
+    return out
+
+
 def cpu_baseline():
     """The reference (oracle/_ref) on a bounded sample: 200^3 x 4 with 125k agents (1/8 of
     the domain, same densities), 8 steps; falls back to the C restatement (1 thread)."""
@@ -135,6 +223,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extra", action="store_true", help="skip the batched and adjoint workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the LOD path is built and measured on a single MI355X "
@@ -169,6 +258,11 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "algorithmic_bytes_per_launch": bytes_per_launch}
     step_gbs = N ** 3 * NSUB * 48 / (f64["ms_per_step"] / 1e3) / 1e9
+    extra = {}
+    if not args.no_extra:
+        extra["batched"] = bench_batched(torch, stream)
+        extra["batched"]["fp32_vs_fp64_dropin"] = extra["batched"]["fp32"]["value"] / f64["value"]
+        extra["adjoint"] = bench_adjoint(torch, stream)
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": "voxel-substrate updates/s per LOD step (FP64 vs FP32/batched) + % HBM peak",
@@ -192,6 +286,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    line.update(extra)
     print(json.dumps(line), flush=True)
 
 

@@ -48,6 +48,10 @@ SIGNATURES = {
     "bfvmb_adjoint_forward": (_c.c_int, [_vp, _c.c_int, _c.c_int]),
     "bfvmb_write_matlab": (_c.c_int, [_vp, _c.c_char_p, _dp]),
     "bfvmb_set_bulk_rates": (_c.c_int, [_vp, _dp, _dp, _dp]),
+    "bfvmb_compute_gradients": (_c.c_int, [_vp]),
+    "bfvmb_download_gradients": (_c.c_int, [_vp, _dp]),
+    "bfvmb_gradients_at": (_c.c_int, [_vp, _c.c_int, _ip, _dp]),
+    "bfvmb_gradient_device_ptr": (_c.c_int, [_vp, _c.POINTER(_vp)]),
     "bfvmb_simulate_bulk_sources_and_sinks": (_c.c_int, [_vp, _c.c_double]),
     "bfvmb_batch_set_bulk_rates": (_c.c_int, [_vp, _c.c_int, _dp, _dp, _dp]),
     "bfvmb_read_matlab": (_c.c_int, [_vp, _c.c_char_p, _dp]),

@@ -119,6 +119,23 @@ class LODSolver:
         self._check(self._lib.bfvmb_set_agent_rates(self._h, n, L.iptr(v), L.dptr(vol), L.dptr(S), L.dptr(U), L.dptr(T),
                                                     None if a is None else L.bptr(a), float(dt)), "set_agent_rates")
 
+    def compute_gradients(self) -> None:
+        """Gradients of the current field into the device buffer (bfvmb_compute_gradients)."""
+        self._check(self._lib.bfvmb_compute_gradients(self._h), "compute_gradients")
+
+    def download_gradients(self) -> np.ndarray:
+        """[x][y][z][substrate][axis] of the last compute_gradients."""
+        out = np.empty(self.shape + (3,), dtype=np.float64)
+        self._check(self._lib.bfvmb_download_gradients(self._h, L.dptr(out)), "download_gradients")
+        return out
+
+    def gradients_at(self, voxel) -> np.ndarray:
+        """[k][substrate][axis] gradients of the current field at the listed voxels."""
+        v = np.ascontiguousarray(voxel, dtype=np.int32)
+        out = np.empty((v.size, self.nsub, 3), dtype=np.float64)
+        self._check(self._lib.bfvmb_gradients_at(self._h, v.size, L.iptr(v), L.dptr(out)), "gradients_at")
+        return out
+
     def set_bulk_rates(self, supply=None, target=None, uptake=None) -> None:
         """Per-voxel bulk supply S, target T, uptake U, each [x][y][z][substrate] (None = 0);
         all None removes the bulk phase (bfvmb_set_bulk_rates)."""

@@ -78,6 +78,8 @@ struct Domain {
     std::vector<int> agent_order;  // sorted position -> index in the caller's agent list
     int agents_given = 0;          // length of the caller's agent list
     double *bulk = nullptr;        // bulk sources/sinks: T*S then U+S, each [nsub][nz][ny][nx]
+    void *grad = nullptr;          // gradients [nsub*3 + axis][nz][ny][nx], field precision
+    bool grad_valid = false;       // grad holds the gradients of some field
 };
 
 // Offsets inside a domain's coefficient block, in doubles.
@@ -154,6 +156,11 @@ int group_autotune(Group *g, int reps, int *chosen);
 int adjoint_forward(Group *g, int nsteps, int every);
 int adjoint_backward(Group *g, const double *seed, double *grad_rho0, double *grad_t1, double *grad_t2);
 void adjoint_free(Group *g);
+// Density gradients of a single-domain group (gradient.hip).
+int domain_compute_gradients(Group *g);
+int domain_download_gradients(Group *g, double *out);
+int domain_gradients_at(Group *g, int n, const int *voxel, double *out);
+
 // MAT v4 files of a single-domain group (matlab.hip).
 int domain_write_matlab(Group *g, const char *path, const double *origin);
 int domain_read_matlab(Group *g, const char *path, const double *origin);

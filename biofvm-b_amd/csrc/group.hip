@@ -232,6 +232,7 @@ void group_free(Group *g)
     for (Domain &d : g->doms) {
         free_agents(d);
         (void)hipFree(d.bulk);
+        (void)hipFree(d.grad);
         (void)hipFree(d.field[0]);
         (void)hipFree(d.field[1]);
         (void)hipFree(d.coef);

@@ -526,10 +526,10 @@ Shape shape_of(const Group *g, int axis)
     const LaunchShape &u = g->launch[axis];
     Shape sh;
     sh.grid = u.grid;
-    // defaults measured on MI355X at 400^3: y/z keep each line in one wavefront's LDS with
-    // a 64-deep ring; the x sweep shares the CU between two wavefronts
-    sh.bpc = u.lds_blocks_per_cu > 0 ? u.lds_blocks_per_cu : (axis == 0 ? 2 : 1);
-    sh.ring = u.ring > 0 ? u.ring : 64;
+    // defaults measured on MI355X at 400^3 x 4: two sweep wavefronts share a CU's LDS, and
+    // y/z keep 32 rows per lane in flight
+    sh.bpc = u.lds_blocks_per_cu > 0 ? u.lds_blocks_per_cu : 2;
+    sh.ring = u.ring > 0 ? u.ring : 32;
     return sh;
 }
 

@@ -156,6 +156,21 @@ int bfvmb_adjoint_forward(bfvmb_ctx *ctx, int nsteps, int checkpoint_every);
 int bfvmb_adjoint_backward(bfvmb_ctx *ctx, const double *seed, double *grad_rho0, double *grad_temp1,
                            double *grad_temp2);
 
+/* ---- Density gradients (Microenvironment::compute_all_gradient_vectors and
+ * compute_gradient_vector, BioFVM_microenvironment.cpp:1243-1533): per voxel, substrate and
+ * axis, (rho[+1] - rho[-1])/(2 dx) inside, one-sided (/dx) at the first and last voxel, 0
+ * along an axis of length 1.  FP64 arithmetic; FP64 contexts match the reference bit for bit.
+ * bfvmb_compute_gradients fills a device buffer from the current field (stream-ordered);
+ * bfvmb_download_gradients copies it out as [voxel][substrate][3] (gradient_vectors'
+ * layout); bfvmb_gradient_device_ptr gives its address ([nsub*3 + axis][z][y][x] of the
+ * field's real type).  bfvmb_gradients_at computes them only at the listed voxels
+ * (reference numbering) from the current field, out[k][substrate][3], as
+ * gradient_vector(n) does for the voxel of one cell. */
+int bfvmb_compute_gradients(bfvmb_ctx *ctx);
+int bfvmb_download_gradients(bfvmb_ctx *ctx, double *grad_voxel_major);
+int bfvmb_gradients_at(bfvmb_ctx *ctx, int n, const int *voxel, double *grad);
+int bfvmb_gradient_device_ptr(bfvmb_ctx *ctx, void **ptr);
+
 /* ---- MAT v4 output and input of the field.
  * bfvmb_write_matlab writes the file Microenvironment::write_to_matlab writes
  * (BioFVM_microenvironment.cpp:1079-1104; MPI-IO variant BioFVM_parallel.cpp:610-678 at

@@ -230,6 +230,41 @@ ORC_API void orc_bulk_sources_sinks(size_t nvox, int nsub, double dt, const doub
     }
 }
 
+/* Density gradients, Microenvironment::compute_all_gradient_vectors
+ * (BioFVM_microenvironment.cpp:1243-1386), grad[voxel][d][axis]:
+ *   interior   (rho[+1] - rho[-1]) / (2*dx)      (two_dx = dx*2.0)
+ *   first      (rho[1] - rho[0]) / dx
+ *   last       (rho[n-1] - rho[n-2]) / dx
+ * each subtraction and division rounded on its own.  An axis of length 1 gives 0 (the
+ * reference skips z then, and reads outside the field for x or y).                     */
+ORC_API void orc_gradients(int nx, int ny, int nz, int nsub, double dx, const double *rho, double *grad)
+{
+    const int n[3] = {nx, ny, nz};
+    const size_t jump[3] = {(size_t)ny * nz * nsub, (size_t)nz * nsub, (size_t)nsub};
+    const double two_dx = dx * 2.0;
+    for (int x = 0; x < nx; ++x)
+        for (int y = 0; y < ny; ++y)
+            for (int z = 0; z < nz; ++z) {
+                const int at[3] = {x, y, z};
+                const size_t v = ((size_t)x * ny + y) * nz + z;
+                for (int d = 0; d < nsub; ++d) {
+                    const double *r = rho + v * nsub + d;
+                    double *g = grad + (v * nsub + d) * 3;
+                    for (int a = 0; a < 3; ++a) {
+                        const ptrdiff_t j = (ptrdiff_t)jump[a];
+                        if (n[a] == 1)
+                            g[a] = 0.0;
+                        else if (at[a] == 0)
+                            g[a] = (r[j] - r[0]) / dx;
+                        else if (at[a] == n[a] - 1)
+                            g[a] = (r[0] - r[-j]) / dx;
+                        else
+                            g[a] = (r[j] - r[-j]) / two_dx;
+                    }
+                }
+            }
+}
+
 /* Agent -> voxel, Basic_Agent::assign_position + update_voxel_index
  * (BioFVM_basic_agent.cpp:112-153) with Cartesian_Mesh::nearest_voxel_local_index at world
  * size 1 (BioFVM_parallel.cpp:418-522) and is_position_valid (BioFVM_mesh.cpp:204-213).

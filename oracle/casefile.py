@@ -98,7 +98,7 @@ def read_output(path: str, case: Case) -> dict:
 
 
 def run_reference(case: Case, workdir: str, solver: str = "lod", granularity: int = 1,
-                  threads: int = 1, timing: bool = False, mat: bool = False) -> dict:
+                  threads: int = 1, timing: bool = False, mat: bool = False, grad: bool = False) -> dict:
     """Run the compiled reference on ``case``; returns its agent mapping and snapshots, and
     with ``mat`` the bytes of Microenvironment::write_to_matlab after the last step."""
     cpath = os.path.join(workdir, "case.bin")
@@ -109,6 +109,9 @@ def run_reference(case: Case, workdir: str, solver: str = "lod", granularity: in
     args = [REF_DRIVER, cpath, opath, solver, str(granularity)] + (["--time"] if timing else [])
     if mat:
         args += ["--mat", mpath]
+    gpath = os.path.join(workdir, "grad.bin")
+    if grad:
+        args += ["--grad", gpath]
     if case.bulk is not None:
         bpath = os.path.join(workdir, "bulk.bin")
         with open(bpath, "wb") as f:
@@ -121,6 +124,8 @@ def run_reference(case: Case, workdir: str, solver: str = "lod", granularity: in
     if mat:
         with open(mpath, "rb") as f:
             out["mat"] = f.read()
+    if grad:
+        out["grad"] = np.fromfile(gpath, dtype=np.float64).reshape(case.nx, case.ny, case.nz, case.nsub, 3)
     return out
 
 

@@ -42,6 +42,7 @@ def lib():
         L.orc_voxel_index.restype = i
         L.orc_run.argtypes = [i, i, i, i, d, d, _dp, _dp, _dp, _bp, i, _ip, _bp, _dp, _dp, i, _dp, _dp]
         L.orc_bulk_sources_sinks.argtypes = [ctypes.c_size_t, i, d, _dp, _dp, _dp, _dp]
+        L.orc_gradients.argtypes = [i, i, i, i, d, _dp, _dp]
         _lib = L
     return _lib
 
@@ -166,6 +167,15 @@ def run(case, nsteps=None, rho=None, with_agents=None, tables=None):
         lib().orc_run(case.nx, case.ny, case.nz, case.nsub, case.dx, case.dt, pD, pdc, pdv, pda, na, pv, pa,
                       p1, p2, 1, rho.ctypes.data_as(_dp), work.ctypes.data_as(_dp))
     return rho
+
+
+def gradients(rho, dx):
+    """[nx][ny][nz][nsub][3] gradients of the AoS field (orc_gradients)."""
+    r = np.ascontiguousarray(rho, dtype=np.float64)
+    nx, ny, nz, ns = r.shape
+    g = np.empty((nx, ny, nz, ns, 3))
+    lib().orc_gradients(nx, ny, nz, ns, dx, r.ctypes.data_as(_dp), g.ctypes.data_as(_dp))
+    return g
 
 
 def bulk_sources_sinks(rho, dt, supply, target, uptake):

@@ -13,7 +13,7 @@
 //   simulate_cell_sources_and_sinks(dt) followed by simulate_diffusion_decay(...).
 //
 // usage: ref_driver <case.bin> <out.bin> [lod|avx] [granularity] [--time] [--mat out.mat]
-//                   [--bulk rates.bin]
+//                   [--bulk rates.bin] [--grad grad.bin]
 //        ref_driver_hip <case.bin> <out.bin> [hip|hipres] ...   (BFVMB_PLUGIN build: the
 //        GPU solver in the diffusion_decay_solver slot; hipres keeps the field resident)
 #include <mpi.h>
@@ -86,6 +86,8 @@ int main(int argc, char **argv)
     int gran = argc > 4 ? atoi(argv[4]) : 1;
     bool timing = argc > 5 && std::string(argv[5]) == "--time";
     std::string mat_path;  // --mat PATH: Microenvironment::write_to_matlab after the last step
+    std::string grad_path;  // --grad PATH: compute_all_gradient_vectors after the last step,
+                            // written as [voxel][substrate][3] doubles
     std::string bulk_path;  // --bulk PATH: supply, target, uptake [nvox][nsub] doubles; per step
                             // simulate_bulk_sources_and_sinks runs before the cell update
     for (int i = 3; i + 1 < argc; ++i) {
@@ -93,6 +95,8 @@ int main(int argc, char **argv)
             mat_path = argv[i + 1];
         if (std::string(argv[i]) == "--bulk")
             bulk_path = argv[i + 1];
+        if (std::string(argv[i]) == "--grad")
+            grad_path = argv[i + 1];
     }
 
     int provided = 0;
@@ -245,6 +249,19 @@ int main(int argc, char **argv)
         }
     }
     fclose(out);
+    if (!grad_path.empty()) {
+#ifdef BFVMB_PLUGIN
+        if (hip)
+            bfvmb_plugin_sync(M);
+#endif
+        M.compute_all_gradient_vectors();
+        FILE *g = fopen(grad_path.c_str(), "wb");
+        if (!g) { perror(grad_path.c_str()); return 2; }
+        for (size_t v = 0; v < nvox; ++v)
+            for (int d = 0; d < nsub; ++d)
+                fwrite(M.gradient_vector((int)v)[d].data(), sizeof(double), 3, g);
+        fclose(g);
+    }
     if (!mat_path.empty()) {
 #ifdef BFVMB_PLUGIN
         if (hip)

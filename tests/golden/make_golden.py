@@ -10,6 +10,7 @@ scalar solver; that is recorded in the fixture.
     python tests/golden/make_golden.py          # the .npz fixtures
     python tests/golden/make_golden.py --mat    # MAT v4 fixtures (<name>.mat + <name>.npz)
     python tests/golden/make_golden.py --bulk   # bulk source/sink fixtures
+    python tests/golden/make_golden.py --grad   # gradient fixtures (field + gradients)
 """
 import os
 import sys
@@ -72,6 +73,23 @@ def bulk_cases():
     yield "bulk_8x1x9x2", c
 
 
+def grad_cases():
+    """Cases whose final field the reference also differentiates
+    (Microenvironment::compute_all_gradient_vectors; every axis of length >= 2, since the
+    reference reads outside the field for an axis of length 1 in x or y)."""
+    yield "grad_19x14x9x2", cf.make_case(19, 14, 9, 2, seed=21, nagents=40, nsteps=3, snaps=[3], dactive=[1, 0])
+    yield "grad_nz1_7x6x1x2", cf.make_case(7, 6, 1, 2, seed=22, nagents=5, nsteps=2, snaps=[2])
+    yield "grad_n2_2x3x2x1", cf.make_case(2, 3, 2, 1, seed=23, nagents=3, nsteps=2, snaps=[2])
+
+
+def main_grad():
+    for name, case in grad_cases():
+        with tempfile.TemporaryDirectory() as d:
+            out = cf.run_reference(case, d, solver="lod", granularity=1, grad=True)
+        path = save(name, case, out, avx_equal=True)
+        print(f"{name}: {os.path.getsize(path) / 1e3:.0f} kB")
+
+
 def main_bulk():
     for name, case in bulk_cases():
         with tempfile.TemporaryDirectory() as d:
@@ -99,6 +117,8 @@ def save(name, case, out, avx_equal):
     if case.bulk is not None:
         for k, a in zip(("bulk_supply", "bulk_target", "bulk_uptake"), case.bulk):
             arrays[k] = a
+    if "grad" in out:
+        arrays["grad"] = out["grad"]
     for s, f in out["fields"].items():
         arrays[f"field_{s}"] = f
     path = os.path.join(HERE, f"{name}.npz")
@@ -127,6 +147,8 @@ def load(name):
                    else None)
     ref = {"voxel": z["voxel"], "temp1": z["temp1"], "temp2": z["temp2"], "avx_equal": bool(int(z["avx_equal"])),
            "fields": {int(s): z[f"field_{int(s)}"] for s in z["snaps"]}}
+    if "grad" in z.files:
+        ref["grad"] = z["grad"]
     return case, ref
 
 
@@ -135,6 +157,7 @@ NAMES = ["config1_50cube", "aniso_17x23x31x3", "nx1_1x5x7x2", "ny1_2x1x9x4", "nz
 
 MAT_NAMES = ["mat_offset_13x9x11x3", "mat_1x1x2x1"]
 BULK_NAMES = ["bulk_15x12x10x3", "bulk_8x1x9x2"]
+GRAD_NAMES = ["grad_19x14x9x2", "grad_nz1_7x6x1x2", "grad_n2_2x3x2x1"]
 ALL_NAMES = NAMES + MAT_NAMES
 
 
@@ -149,5 +172,7 @@ if __name__ == "__main__":
         main_mat()
     elif "--bulk" in sys.argv[1:]:
         main_bulk()
+    elif "--grad" in sys.argv[1:]:
+        main_grad()
     else:
         main()

@@ -78,7 +78,7 @@ def test_loss_and_field_restored():
     assert np.array_equal(h0, g0) and np.array_equal(h1, g1) and np.array_equal(h2, g2)
     # linear in the seed
     k0, _, _ = s.adjoint_backward(-3.0 * (rT - tgt))
-    assert np.allclose(k0, -1.5 * g0, rtol=1e-14, atol=0)
+    assert np.linalg.norm(k0 + 1.5 * g0) <= 1e-13 * np.linalg.norm(g0)
     # more steps invalidate the record; the context keeps stepping exactly
     s.run(2)
     with pytest.raises(Exception, match="adjoint_backward"):

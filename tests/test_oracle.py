@@ -202,3 +202,33 @@ def test_bulk_closed_forms():
     for _ in range(4000):
         orc.bulk_sources_sinks(r, 0.5, S, T, z)
     assert np.allclose(r, T, rtol=1e-12, atol=0)
+
+
+# ---------------------------------------------------------------- gradients
+@pytest.mark.parametrize("name", golden.GRAD_NAMES)
+def test_gradients_match_reference_fixtures(name):
+    """orc_gradients bit for bit against compute_all_gradient_vectors run by the reference."""
+    case, ref = golden.load(name)
+    rho = ref["fields"][case.nsteps]
+    assert np.array_equal(orc.gradients(rho, case.dx), ref["grad"])
+
+
+def test_gradients_exact_on_linear_and_quadratic_fields():
+    """Centered differences are exact for quadratics in the interior, one-sided ones for
+    linear fields at the ends (grid values chosen to be exact in binary)."""
+    nx, ny, nz, dx = 6, 5, 4, 0.5
+    x, y, z = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    lin = (3.0 * x - 2.0 * y + 5.0 * z)[..., None] * 1.0
+    g = orc.gradients(lin, dx)
+    assert np.all(g[..., 0, 0] == 3.0 / dx) and np.all(g[..., 0, 1] == -2.0 / dx) and np.all(g[..., 0, 2] == 5.0 / dx)
+    quad = (x * x * 1.0)[..., None]
+    g = orc.gradients(quad, dx)
+    assert np.array_equal(g[1:-1, ..., 0, 0], (2.0 * x[1:-1]) / dx)  # d(x^2)/dx = 2x exactly
+    assert np.all(g[0, ..., 0, 0] == 1.0 / dx) and np.all(g[-1, ..., 0, 0] == (2 * nx - 3) / dx)
+    assert np.all(g[..., 0, 1:] == 0.0)
+
+
+def test_gradients_axis_of_length_one_is_zero():
+    rho = np.random.default_rng(0).uniform(size=(1, 4, 1, 2))
+    g = orc.gradients(rho, 20.0)
+    assert np.all(g[..., 0] == 0.0) and np.all(g[..., 2] == 0.0) and np.any(g[..., 1] != 0.0)
