@@ -145,6 +145,34 @@ def bench_batched(torch, stream, steps=20, seed=3):
     return out
 
 
+def bench_c5(torch, stream, steps=10, nagents=10_000_000, seed=9):
+    """FP32 bulk path at 640^3 x 8 substrates (log-spaced D and decay, Dirichlet on
+    substrates 0-1, 10M agents in a centred 5 mm sphere), constant initial field."""
+    from biofvmb import LODSolver
+    n, ns = 640, 8
+    rng = np.random.default_rng(seed)
+    Dm = np.logspace(2, 5, ns)
+    lam = np.logspace(-3, 0, ns)
+    s = LODSolver(n, n, n, ns, DX, "fp32")
+    s.set_stream(stream)
+    s.set_coefficients(Dm, lam, DT)
+    s.set_dirichlet(np.full(ns, 38.0), (np.arange(ns) < 2).astype(np.uint8))
+    s.upload(np.ones((n, n, n, ns)))
+    d = rng.normal(size=(nagents, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    pos = n * DX / 2.0 + d * (5000.0 * rng.uniform(0, 1, nagents) ** (1.0 / 3.0))[:, None]
+    vox = s.voxel_indices(pos)
+    vox[vox < 0] = 0
+    s.set_agent_rates(vox, np.full(nagents, 4.0 / 3.0 * np.pi * 125.0), rng.uniform(0, 5, (nagents, ns)),
+                      rng.uniform(0, 20, (nagents, ns)), np.ones((nagents, ns)), DT)
+    gpu_s, wall = time_steps(s, torch, steps, 2)
+    s.close()
+    vs = n ** 3 * ns * steps / gpu_s
+    log(f"c5 fp32: {vs:.4e} vs/s, {gpu_s * 1e3 / steps:.3f} ms/step")
+    return {"workload": f"FP32 {n}^3x{ns}, {nagents} agents", "value": vs, "ms_per_step": gpu_s * 1e3 / steps,
+            "step_hbm_frac": round(n ** 3 * ns * 24 / (gpu_s / steps) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
 def bench_adjoint(torch, stream, n=128, nsteps=50, every=10, nagents=10_000, seed=5):
     """Gradient of sum((rho_T - target)^2) after 50 steps at 128^3 x 2 with 10k agents:
     forward run with checkpoints every 10 steps, then the reverse pass."""
@@ -224,6 +252,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and adjoint workloads")
+    ap.add_argument("--c5", action="store_true", help="also time the FP32 640^3 x 8 workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the LOD path is built and measured on a single MI355X "
@@ -263,6 +292,8 @@ def main():
         extra["batched"] = bench_batched(torch, stream)
         extra["batched"]["fp32_vs_fp64_dropin"] = extra["batched"]["fp32"]["value"] / f64["value"]
         extra["adjoint"] = bench_adjoint(torch, stream)
+    if args.c5:
+        extra["fp32_640cube_x8"] = bench_c5(torch, stream)
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": "voxel-substrate updates/s per LOD step (FP64 vs FP32/batched) + % HBM peak",

@@ -588,11 +588,12 @@ int group_autotune(Group *g, int reps, int *chosen)
     for (int axis = 0; axis < 3 && !rc; ++axis) {
         float best = 1e30f;
         LaunchShape best_shape;
-        const int rings[3] = {16, 32, 64};
-        for (int bpc = 1; bpc <= 3 && !rc; ++bpc)
-            for (int r = 0; r < (axis == 0 ? 1 : 3) && !rc; ++r) {
+        const int rings[2] = {32, 64};
+        const int bpcs[5] = {2, 3, 4, 6, 8};
+        for (int b = 0; b < 5 && !rc; ++b)
+            for (int r = 0; r < (axis == 0 ? 1 : 2) && !rc; ++r) {
                 g->launch[axis] = LaunchShape();
-                g->launch[axis].lds_blocks_per_cu = bpc;
+                g->launch[axis].lds_blocks_per_cu = bpcs[b];
                 g->launch[axis].ring = axis == 0 ? 0 : rings[r];
                 if ((rc = launch_sweep(g, axis)))  // warm-up
                     break;

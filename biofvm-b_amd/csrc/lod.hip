@@ -10,10 +10,12 @@
 // Design (the field is [substrate][z][y][x], x contiguous; every sweep reads one buffer and
 // writes the other):
 //  * One wavefront owns 64 lines.  The forward results of the tail of every line are held
-//    in a per-wavefront LDS "line store" (up to the whole 160 KB of the CU), only the head
-//    of a line longer than the store is parked in the output buffer, where it stays in the
-//    XCD's L2.  The backward pass then reads the tail from LDS.  HBM therefore sees one
-//    read and one write per voxel and sweep, the algorithmic minimum.
+//    in a per-wavefront LDS "line store" (a share of the CU's 160 KB); the backward pass
+//    reads them from there.  The head of a line longer than its store is only swept in the
+//    forward pass, keeping the recurrence value at each segment end in the output row the
+//    backward pass overwrites last; the backward pass then recomputes each head segment
+//    from its checkpoint (re-reading the input, bit-identical arithmetic) into the freed
+//    store.  HBM sees one read and one write per voxel, plus one re-read of the head.
 //  * y and z sweeps: lane = line, consecutive lanes on consecutive x (y sweep) or xy
 //    (z sweep), so each step of the recurrence is a coalesced 512 B row access; a ring of
 //    P rows per lane is kept in flight.
@@ -73,14 +75,12 @@ __device__ __forceinline__ double fwd(double c1, double prev, double v, double d
     }
 }
 
-// Dispatch of the first P-aligned boundary: the forward pass parks [0, lo) in the output
-// buffer and keeps [lo, n) in LDS; lo is a multiple of P unless every element is parked.
+// First row of a line kept in the line store of L rows: the forward pass keeps [lo, n) in
+// LDS and recomputes the head [0, lo) later; lo is a multiple of P (L >= P is required).
 __host__ __device__ __forceinline__ int park_boundary(int n, int L, int P)
 {
     if (L >= n)
         return 0;
-    if (L < P)
-        return n;
     const int lo = (n - L + P - 1) / P * P;
     return lo < n ? lo : n;
 }
@@ -185,18 +185,24 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
         const bool face = act && (x == 0 || x == nx - 1 || o == 0 || o == on - 1);
         const double c1 = D.c1[s];
 
-        // ---- forward elimination; a ring of P rows is in flight (reads past the end
-        // re-read row n-1 and are discarded)
+        // ---- forward elimination; a ring of P rows is in flight.  The head [0, lo) is only
+        // swept here (its results are recomputed segment by segment in the backward pass,
+        // from checkpoints kept at the segment ends); the tail [lo, n) goes to the line store.
+        const int S = L / P * P;  // head segment length (the host guarantees L >= P)
+        const int nhead = lo > 0 ? (lo + S - 1) / S : 0;
+        const int r0 = lo - (nhead - 1) * S;  // end of the first head segment
         T buf[P];
 #pragma unroll
         for (int j = 0; j < P; ++j)
             buf[j] = src[(size_t)min(j, n - 1) * stride];
         double y = -0.0;
         double amin = 1.0;
-        int k0 = 0;
-        // one full chunk of P elements; PARK: results to the output buffer, else to LDS
-        auto chunk = [&](auto park_tag) {
-            constexpr bool PARK = decltype(park_tag)::value;
+        // one full chunk of P rows from k0: prefetches row min(k + P, lim) (PF), stores the
+        // results in the line store at row k - base (STORE)
+        auto chunk = [&](int k0, int base, int lim, auto store_tag, auto exact_tag, auto pf_tag) {
+            constexpr bool STORE = decltype(store_tag)::value;
+            constexpr bool EXACT = decltype(exact_tag)::value;
+            constexpr bool PF = decltype(pf_tag)::value;
 #pragma unroll
             for (int jb = 0; jb < P; jb += TB) {
                 double2 t[TB];
@@ -208,52 +214,54 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
                     const int j = jb + jj;
                     const int k = k0 + j;
                     const double v = (double)buf[j];
-                    buf[j] = src[(size_t)min(k + P, n - 1) * stride];
-                    y = fwd<T>(c1, y, v, t[jj].x, t[jj].y, amin);
-                    if constexpr (PARK)
-                        dst[(size_t)k * stride] = face ? dv : (T)y;
+                    if constexpr (PF)
+                        buf[j] = src[(size_t)min(k + P, lim) * stride];
+                    if constexpr (EXACT)
+                        y = fma(c1, y, v) / t[jj].x;
                     else
-                        ls[(size_t)(k - lo) * kWave + lane] = (T)y;
+                        y = fwd<T>(c1, y, v, t[jj].x, t[jj].y, amin);
+                    if constexpr (STORE)
+                        ls[(size_t)(k - base) * kWave + lane] = (T)y;
                 }
             }
         };
-        // the last, partial chunk of a phase (uniform per-element conditions)
-        auto tail = [&](int end, auto park_tag) {
-            constexpr bool PARK = decltype(park_tag)::value;
+        const std::false_type no_store, inexact, no_pf;
+        const std::true_type store, exact_div, pf;
+        int k0 = 0;
+        for (int next_ck = r0; k0 < lo; k0 += P) {  // head: checkpoints only
+            chunk(k0, 0, n - 1, no_store, inexact, pf);
+            if (k0 + P == next_ck && next_ck < lo) {
+                dst[(size_t)(next_ck - 1) * stride] = (T)y;
+                next_ck += S;
+            }
+        }
+        for (; k0 + P <= n; k0 += P)  // tail: line store
+            chunk(k0, lo, n - 1, store, inexact, pf);
+        if (k0 < n) {  // the last, partial chunk (uniform per-row conditions)
 #pragma unroll
             for (int j = 0; j < P; ++j) {
                 const int k = k0 + j;
-                if (k < end) {
+                if (k < n) {
                     const double2 t = tab2[k];
                     y = fwd<T>(c1, y, (double)buf[j], t.x, t.y, amin);
-                    if constexpr (PARK)
-                        dst[(size_t)k * stride] = face ? dv : (T)y;
-                    else
-                        ls[(size_t)(k - lo) * kWave + lane] = (T)y;
+                    ls[(size_t)(k - lo) * kWave + lane] = (T)y;
                 }
             }
-        };
-        for (; k0 + P <= lo; k0 += P)  // [0, lo): parked in the output buffer
-            chunk(std::true_type());
-        if (k0 < lo) {  // only when every element is parked and n is not a multiple of P
-            tail(lo, std::true_type());
-            k0 = lo;
         }
-        for (; k0 + P <= n; k0 += P)  // [lo, n): LDS line store
-            chunk(std::false_type());
-        if (k0 < n)
-            tail(n, std::false_type());
+        bool exact = false;
         if constexpr (sizeof(T) == 8) {
             // a numerator below 2^-960 somewhere in the tile: redo the forward pass with
-            // true divisions (same results everywhere else)
+            // true divisions (same results everywhere else); the head recomputation below
+            // then divides the same way
             if (__builtin_expect(__any(amin < kTiny), 0)) {
+                exact = true;
                 y = -0.0;
                 for (int k = 0; k < n; ++k) {
                     y = fma(c1, y, (double)src[(size_t)k * stride]) / tab2[k].x;
                     if (k >= lo)
                         ls[(size_t)(k - lo) * kWave + lane] = (T)y;
-                    else
-                        dst[(size_t)k * stride] = face ? dv : (T)y;
+                    else if (k + 1 >= r0 && k + 1 < lo && (k + 1 - r0) % S == 0)
+                        dst[(size_t)k * stride] = (T)y;
                 }
             }
         }
@@ -261,67 +269,70 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
         // ---- back substitution; the last element is y[n-1] itself
         double xv = y;
         dst[(size_t)(n - 1) * stride] = act ? dv : (T)xv;
-        // LDS part, k = n-2 .. lo, in batches of TB reads
-        for (int kt = n - 2; kt >= lo; kt -= TB) {
-            if (kt - (TB - 1) >= lo) {
-                double v[TB], c[TB];
+        // rows [base, top] from the line store (row k at k - base), in batches of TB reads
+        auto back = [&](int top, int base) {
+            for (int kt = top; kt >= base; kt -= TB) {
+                if (kt - (TB - 1) >= base) {
+                    double v[TB], c[TB];
 #pragma unroll
-                for (int j = 0; j < TB; ++j) {
-                    v[j] = (double)ls[(size_t)(kt - j - lo) * kWave + lane];
-                    c[j] = tabc[kt - j];
-                }
+                    for (int j = 0; j < TB; ++j) {
+                        v[j] = (double)ls[(size_t)(kt - j - base) * kWave + lane];
+                        c[j] = tabc[kt - j];
+                    }
 #pragma unroll
-                for (int j = 0; j < TB; ++j) {
-                    const int k = kt - j;
-                    xv = fma(-c[j], xv, v[j]);
-                    dst[(size_t)k * stride] = (face || (act && k == 0)) ? dv : (T)xv;
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < TB; ++j) {
-                    const int k = kt - j;
-                    if (k >= lo) {
-                        xv = fma(-tabc[k], xv, (double)ls[(size_t)(k - lo) * kWave + lane]);
+                    for (int j = 0; j < TB; ++j) {
+                        const int k = kt - j;
+                        xv = fma(-c[j], xv, v[j]);
                         dst[(size_t)k * stride] = (face || (act && k == 0)) ? dv : (T)xv;
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < TB; ++j) {
+                        const int k = kt - j;
+                        if (k >= base) {
+                            xv = fma(-tabc[k], xv, (double)ls[(size_t)(k - base) * kWave + lane]);
+                            dst[(size_t)k * stride] = (face || (act && k == 0)) ? dv : (T)xv;
+                        }
                     }
                 }
             }
-        }
-        // parked head, k = min(lo-1, n-2) .. 0, read back through the ring
-        const int top = min(lo - 1, n - 2);
-        if (top >= 0) {
+        };
+        // head segments, last first: recompute the forward results from the checkpoint at
+        // the row before the segment (re-reading the input), then substitute back.  The ring
+        // and checkpoint of the next segment are loaded while the current one substitutes back.
+        double yck = -0.0;
+        auto seg_begin = [&](int b) { return b > r0 ? b - S : 0; };
+        auto prefetch_segment = [&](int b) {
+            const int a = seg_begin(b);
+            yck = a == 0 ? -0.0 : (double)dst[(size_t)(a - 1) * stride];
 #pragma unroll
             for (int j = 0; j < P; ++j)
-                buf[j] = dst[(size_t)max(top - j, 0) * stride];
-            int kt = top;
-            for (; kt - (P - 1) >= 0; kt -= P) {
-#pragma unroll
-                for (int jb = 0; jb < P; jb += TB) {
-                    double c[TB];
-#pragma unroll
-                    for (int jj = 0; jj < TB; ++jj)
-                        c[jj] = tabc[kt - jb - jj];
-#pragma unroll
-                    for (int jj = 0; jj < TB; ++jj) {
-                        const int j = jb + jj;
-                        const int k = kt - j;
-                        const double v = (double)buf[j];
-                        buf[j] = dst[(size_t)max(k - P, 0) * stride];
-                        xv = fma(-c[jj], xv, v);
-                        dst[(size_t)k * stride] = (face || (act && k == 0)) ? dv : (T)xv;
-                    }
+                buf[j] = src[(size_t)min(a + j, b - 1) * stride];
+        };
+        if (lo > 0)
+            prefetch_segment(lo);
+        back(n - 2, lo);
+        for (int b = lo; b > 0;) {
+            const int a = seg_begin(b);
+            y = yck;
+            for (int kk = a; kk < b; kk += P) {
+                const bool last = kk + P >= b;
+                if (exact) {
+                    if (last)
+                        chunk(kk, a, b - 1, store, exact_div, no_pf);
+                    else
+                        chunk(kk, a, b - 1, store, exact_div, pf);
+                } else {
+                    if (last)
+                        chunk(kk, a, b - 1, store, inexact, no_pf);
+                    else
+                        chunk(kk, a, b - 1, store, inexact, pf);
                 }
             }
-            if (kt >= 0) {
-#pragma unroll
-                for (int j = 0; j < P; ++j) {
-                    const int k = kt - j;
-                    if (k >= 0) {
-                        xv = fma(-tabc[k], xv, (double)buf[j]);
-                        dst[(size_t)k * stride] = (face || (act && k == 0)) ? dv : (T)xv;
-                    }
-                }
-            }
+            if (a > 0)
+                prefetch_segment(a);
+            back(b - 1, a);
+            b = a;
         }
     }
 }
@@ -399,91 +410,98 @@ __global__ __launch_bounds__(64) void k_sweep_x(const DomainDesc<T> *__restrict_
             }
         };
 
-        // ---- forward; EXACT = true divisions (the rare redo for numerators < 2^-960)
+        // ---- forward.  Head chunks [0, cl) are swept without storing their results (the
+        // backward pass recomputes them segment by segment from checkpoints kept at the
+        // segment ends); the tail chunks go to the line store.
+        const int Sc = L / W;  // head segment length in chunks (the host guarantees L >= W)
+        const int nhead = cl > 0 ? (cl + Sc - 1) / Sc : 0;
+        const int r0c = cl - (nhead - 1) * Sc;  // end of the first head segment
+        const size_t my_off = (size_t)min(lane, rows_here - 1) * nx;  // this lane's row
         T ring[PC][NI];
         double y = -0.0;
-        auto forward = [&](auto exact_tag) {
+        // chunk c: stage the ring slot, refill it with chunk `next`, then W recurrence steps;
+        // results to the line store at chunk c - base when STORE
+        auto fwd_chunk = [&](int c, T *slot, int next, int base, auto store_tag, auto exact_tag, double &amin) {
+            constexpr bool STORE = decltype(store_tag)::value;
             constexpr bool EXACT = decltype(exact_tag)::value;
+            put_chunk(slot);
+            if (next >= 0)  // next < 0: nothing left to prefetch into this slot
+                load_chunk(src, next, slot);
+            __syncthreads();
+            const int w = min(W, nx - c * W);
+            double dd[W], rd[W], v[W];
+#pragma unroll
+            for (int j = 0; j < W; ++j) {
+                const int kk = min(c * W + j, nx - 1);
+                dd[j] = den[kk];
+                rd[j] = rden[kk];
+                const double u = (double)stg[lane * (W + 1) + j];
+                v[j] = (act && (kk == 0 || kk == nx - 1)) ? (double)dv : u;
+            }
+            T *tgt = ls + lane * Lp + (c - base) * W;
+            auto step = [&](int j) {
+                if constexpr (EXACT)
+                    y = fma(c1, y, v[j]) / dd[j];
+                else
+                    y = fwd<T>(c1, y, v[j], dd[j], rd[j], amin);
+                if constexpr (STORE)
+                    tgt[j] = (T)y;
+            };
+            if (w == W) {
+#pragma unroll
+                for (int j = 0; j < W; ++j)
+                    step(j);
+            } else {
+#pragma unroll
+                for (int j = 0; j < W; ++j)
+                    if (j < w)
+                        step(j);
+            }
+            __syncthreads();
+        };
+        const std::false_type no_store, inexact;
+        const std::true_type store, exact_div;
+        auto forward = [&](auto exact_tag) {
             double amin = 1.0;
 #pragma unroll
             for (int p = 0; p < PC; ++p)
                 load_chunk(src, min(p, nch - 1), ring[p]);
             y = -0.0;
+            int next_ck = r0c;
             for (int c0 = 0; c0 < nch; c0 += PC) {
 #pragma unroll
                 for (int p = 0; p < PC; ++p) {
                     const int c = c0 + p;
                     if (c < nch) {
-                        put_chunk(ring[p]);
-                        load_chunk(src, min(c + PC, nch - 1), ring[p]);
-                        __syncthreads();
-                        const int w = min(W, nx - c * W);
-                        double dd[W], rd[W], v[W];
-#pragma unroll
-                        for (int j = 0; j < W; ++j) {
-                            const int kk = min(c * W + j, nx - 1);
-                            dd[j] = den[kk];
-                            rd[j] = rden[kk];
-                            const double u = (double)stg[lane * (W + 1) + j];
-                            v[j] = (act && (kk == 0 || kk == nx - 1)) ? (double)dv : u;
-                        }
-                        // forward results go to the line store or, for parked chunks, back
-                        // into the staging tile
-                        T *tgt = c >= cl ? ls + lane * Lp + (c - cl) * W : stg + lane * (W + 1);
-                        auto step = [&](int j) {
-                            if constexpr (EXACT)
-                                y = fma(c1, y, v[j]) / dd[j];
-                            else
-                                y = fwd<T>(c1, y, v[j], dd[j], rd[j], amin);
-                            tgt[j] = (T)y;
-                        };
-                        if (w == W) {
-#pragma unroll
-                            for (int j = 0; j < W; ++j)
-                                step(j);
-                        } else {
-#pragma unroll
-                            for (int j = 0; j < W; ++j)
-                                if (j < w)
-                                    step(j);
-                        }
-                        __syncthreads();
-                        if (c < cl) {
-                            store_chunk(c);  // park the forward results of the head
-                            __syncthreads();
+                        if (c < cl)
+                            fwd_chunk(c, ring[p], min(c + PC, nch - 1), 0, no_store, exact_tag, amin);
+                        else
+                            fwd_chunk(c, ring[p], min(c + PC, nch - 1), cl, store, exact_tag, amin);
+                        if (c + 1 == next_ck && next_ck < cl) {  // checkpoint at a segment end
+                            dst[my_off + (size_t)next_ck * W - 1] = (T)y;
+                            next_ck += Sc;
                         }
                     }
                 }
             }
             return amin;
         };
-        const double amin = forward(std::integral_constant<bool, false>());
+        bool exact = false;
+        const double amin = forward(inexact);
         if constexpr (sizeof(T) == 8) {
-            if (__builtin_expect(__any(amin < kTiny), 0))
-                forward(std::integral_constant<bool, true>());
+            if (__builtin_expect(__any(amin < kTiny), 0)) {
+                exact = true;
+                forward(exact_div);
+            }
         } else {
             (void)amin;
         }
 
-        // ---- backward, last chunk first; parked chunks come back through the ring
+        // ---- backward from the line store: chunk c's forward results sit at c - base
         double xv = y;
-#pragma unroll
-        for (int p = 0; p < PC; ++p)
-            load_chunk(dst, max(cl - 1 - p, 0), ring[p]);
-        int pslot = 0;
-        for (int c = nch - 1; c >= 0; --c) {
-            if (c < cl) {
-#pragma unroll
-                for (int p = 0; p < PC; ++p)
-                    if (p == pslot) {
-                        put_chunk(ring[p]);
-                        load_chunk(dst, max(c - PC, 0), ring[p]);
-                    }
-                pslot = pslot + 1 == PC ? 0 : pslot + 1;
-            }
-            __syncthreads();
+        auto back_chunk = [&](int c, int base) {
             const int w = min(W, nx - c * W);
-            const T *from = c >= cl ? ls + lane * Lp + (c - cl) * W : stg + lane * (W + 1);
+            const T *from = ls + lane * Lp + (c - base) * W;
             double cw[W], v[W];
 #pragma unroll
             for (int j = 0; j < W; ++j) {
@@ -509,6 +527,45 @@ __global__ __launch_bounds__(64) void k_sweep_x(const DomainDesc<T> *__restrict_
             __syncthreads();
             store_chunk(c);
             __syncthreads();
+        };
+        // head segments, last first: recompute from the checkpoint (re-reading the input),
+        // then substitute back; the next segment's ring and checkpoint are loaded while the
+        // current one substitutes back
+        double yck = -0.0;
+        auto seg_begin = [&](int cb) { return cb > r0c ? cb - Sc : 0; };
+        auto prefetch_segment = [&](int cb) {
+            const int ca = seg_begin(cb);
+            yck = ca == 0 ? -0.0 : (double)dst[my_off + (size_t)ca * W - 1];
+#pragma unroll
+            for (int p = 0; p < PC; ++p)
+                load_chunk(src, min(ca + p, cb - 1), ring[p]);
+        };
+        if (cl > 0)
+            prefetch_segment(cl);
+        for (int c = nch - 1; c >= cl; --c)
+            back_chunk(c, cl);
+        for (int cb = cl; cb > 0;) {
+            const int ca = seg_begin(cb);
+            y = yck;
+            double unused = 1.0;
+            for (int c0 = ca; c0 < cb; c0 += PC) {
+#pragma unroll
+                for (int p = 0; p < PC; ++p) {
+                    const int c = c0 + p;
+                    if (c < cb) {
+                        const int next = c + PC < cb ? c + PC : -1;
+                        if (exact)
+                            fwd_chunk(c, ring[p], next, ca, store, exact_div, unused);
+                        else
+                            fwd_chunk(c, ring[p], next, ca, store, inexact, unused);
+                    }
+                }
+            }
+            if (ca > 0)
+                prefetch_segment(ca);
+            for (int c = cb - 1; c >= ca; --c)
+                back_chunk(c, ca);
+            cb = ca;
         }
     }
 }
@@ -526,9 +583,9 @@ Shape shape_of(const Group *g, int axis)
     const LaunchShape &u = g->launch[axis];
     Shape sh;
     sh.grid = u.grid;
-    // defaults measured on MI355X at 400^3 x 4: two sweep wavefronts share a CU's LDS, and
-    // y/z keep 32 rows per lane in flight
-    sh.bpc = u.lds_blocks_per_cu > 0 ? u.lds_blocks_per_cu : 2;
+    // defaults measured on MI355X at 400^3 x 4 (with the head recomputation): four sweep
+    // wavefronts share a CU's LDS, and y/z keep 32 rows per lane in flight
+    sh.bpc = u.lds_blocks_per_cu > 0 ? u.lds_blocks_per_cu : 4;
     sh.ring = u.ring > 0 ? u.ring : 32;
     return sh;
 }
@@ -553,20 +610,50 @@ int launch_sweep_t(Group *g, int axis)
         return rc;
     const int ndom = (int)g->doms.size();
     const Shape sh = shape_of(g, axis);
-    const size_t budget = std::min(kLdsBytes / sh.bpc, g->max_lds_per_block);
-    // LDS actually needed by the largest domain, capped by the budget
+    size_t budget = std::min(kLdsBytes / sh.bpc, g->max_lds_per_block);
+    // LDS actually needed by the largest domain, capped by the budget; the budget is raised
+    // (fewer wavefronts per CU) where a line store would hold less than one 16-row ring or
+    // one 16-column chunk
     size_t need = 0;
     for (const Domain &d : g->doms) {
+        size_t full, floor;
         if (axis == 0) {
-            need = std::max(need, (size_t)kWave * 17 * sizeof(T) + (size_t)kWave * ((d.nx + 15) / 16 * 16 + 1) * sizeof(T));
+            const size_t stg = (size_t)kWave * 17 * sizeof(T);
+            full = stg + (size_t)kWave * ((d.nx + 15) / 16 * 16 + 1) * sizeof(T);
+            floor = std::min(full, stg + (size_t)kWave * 17 * sizeof(T));
         } else {
             const int n = axis == 1 ? d.ny : d.nz;
-            if ((size_t)n * 24 + 16 > budget)
-                return set_error(g, BFVMB_ERR_ARG, "sweep: axis too long for the LDS Thomas tables");
-            need = std::max(need, (size_t)n * 24 + (size_t)kWave * n * sizeof(T));
+            full = (size_t)n * 24 + (size_t)kWave * n * sizeof(T);
+            floor = std::min(full, (size_t)n * 24 + (size_t)kWave * 16 * sizeof(T));
         }
+        if (floor > g->max_lds_per_block)
+            return set_error(g, BFVMB_ERR_ARG, "sweep: axis too long for the LDS line store");
+        budget = std::max(budget, floor);
+        need = std::max(need, full);
     }
     const size_t lds = std::min(budget, need);
+    int ring = sh.ring;
+    if (axis == 0) {
+        for (const Domain &d : g->doms)
+            if (x_line_store<T>(d.nx, (long long)lds) < 16)
+                return set_error(g, BFVMB_ERR_ARG, "sweep: LDS budget too small for the x line store");
+    } else {
+        // a line longer than its line store is recomputed in segments of whole rings, so the
+        // store must hold at least one ring: take the deepest ring (<= the requested one)
+        // that fits every such domain
+        int minL = 1 << 30;
+        for (const Domain &d : g->doms) {
+            const int n = axis == 1 ? d.ny : d.nz;
+            const int L = yz_line_store<T>(n, (long long)lds);
+            if (L < n)
+                minL = std::min(minL, L);
+        }
+        ring = ring >= 64 ? 64 : ring >= 32 ? 32 : 16;
+        while (ring > 16 && ring > minL)
+            ring /= 2;
+        if (minL < ring)
+            return set_error(g, BFVMB_ERR_ARG, "sweep: LDS budget too small for the y/z line store");
+    }
     const long long *pre = g->d_prefix + (size_t)(axis + 1) * (ndom + 1);
     const long long ntiles = g->prefix[axis + 1][ndom];
     if (ntiles == 0)
@@ -578,9 +665,9 @@ int launch_sweep_t(Group *g, int axis)
         hipLaunchKernelGGL((k_sweep_x<T, 3>), dim3((unsigned)grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur,
                            (int)lds);
     else if (axis == 1)
-        launch_yz<T, 1>(g, d, pre, ndom, (unsigned)grid, lds, sh.ring);
+        launch_yz<T, 1>(g, d, pre, ndom, (unsigned)grid, lds, ring);
     else
-        launch_yz<T, 2>(g, d, pre, ndom, (unsigned)grid, lds, sh.ring);
+        launch_yz<T, 2>(g, d, pre, ndom, (unsigned)grid, lds, ring);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return hip_check(g, e, "sweep launch");

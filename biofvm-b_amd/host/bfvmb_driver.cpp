@@ -10,6 +10,7 @@
 // followed by simulate_diffusion_decay(dt).
 //
 // usage: bfvmb_driver <case.bin> <out.bin> [fp64|fp32] [--fused] [--time] [--mat out.mat]
+//                     [--bulk rates.bin]
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -35,6 +36,16 @@ T one(FILE *f)
     get(f, &v, 1);
     return v;
 }
+
+// per-voxel bulk rates served through the bulk_*_function slots
+std::vector<double> g_bulk[3];
+int g_bulk_nsub = 0;
+template <int K>
+void bulk_rate(biofvmb::Microenvironment *, int v, double *dst)
+{
+    for (int p = 0; p < g_bulk_nsub; ++p)
+        dst[p] = g_bulk[K][(size_t)v * g_bulk_nsub + p];
+}
 }  // namespace
 
 int main(int argc, char **argv)
@@ -45,10 +56,13 @@ int main(int argc, char **argv)
     }
     int precision = BFVMB_FP64;
     bool fused = false, timing = false;
-    const char *mat_path = nullptr;  // --mat PATH: write_to_matlab after the last step
+    const char *mat_path = nullptr;   // --mat PATH: write_to_matlab after the last step
+    const char *bulk_path = nullptr;  // --bulk PATH: supply, target, uptake [nvox][nsub] doubles
     for (int i = 3; i < argc; ++i) {
         if (!std::strcmp(argv[i], "--mat") && i + 1 < argc)
             mat_path = argv[++i];
+        else if (!std::strcmp(argv[i], "--bulk") && i + 1 < argc)
+            bulk_path = argv[++i];
         else if (!std::strcmp(argv[i], "fp32"))
             precision = BFVMB_FP32;
         else if (!std::strcmp(argv[i], "--fused"))
@@ -139,6 +153,24 @@ int main(int argc, char **argv)
         std::fwrite(t1.data(), sizeof(double), t1.size(), out);
         std::fwrite(t2.data(), sizeof(double), t2.size(), out);
         std::fwrite(&nsnap, sizeof(int), 1, out);
+        if (bulk_path) {
+            FILE *b = std::fopen(bulk_path, "rb");
+            if (!b) {
+                std::perror(bulk_path);
+                return 2;
+            }
+            g_bulk_nsub = nsub;
+            for (auto &v : g_bulk) {
+                v.resize(nvox * nsub);
+                get(b, v.data(), v.size());
+            }
+            std::fclose(b);
+            M.bulk_supply_rate_function = bulk_rate<0>;
+            M.bulk_supply_target_densities_function = bulk_rate<1>;
+            M.bulk_uptake_rate_function = bulk_rate<2>;
+            if (fused)
+                M.update_rates();  // constant rates: run() applies them every step
+        }
         int next = 0, done = 0;
         double ms = 0.0;
         for (int s = 1; s <= nsteps; ++s) {
@@ -148,13 +180,16 @@ int main(int argc, char **argv)
             const int k = s - done;
             auto ta = std::chrono::steady_clock::now();
             if (fused) {
-                if (with_agents)
-                    M.run(biofvmb::all_basic_agents, dt, k);
+                std::vector<biofvmb::Basic_Agent *> none;
+                if (with_agents || bulk_path)
+                    M.run(with_agents ? biofvmb::all_basic_agents : none, dt, k);
                 else
                     for (int i = 0; i < k; ++i)
                         M.simulate_diffusion_decay(dt);
             } else {
                 for (int i = 0; i < k; ++i) {
+                    if (bulk_path)
+                        M.simulate_bulk_sources_and_sinks(dt);
                     if (with_agents)
                         M.simulate_cell_sources_and_sinks(dt);
                     M.simulate_diffusion_decay(dt);

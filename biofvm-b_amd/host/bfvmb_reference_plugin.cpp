@@ -40,6 +40,7 @@ struct PluginState {
     std::vector<uint8_t> dact;
     std::vector<double> agent_volume;  // volumes seen at the last upload
     const std::vector<Basic_Agent *> *agents_key = nullptr;
+    std::vector<double> bulk[3];       // bulk supply / target / uptake at the last upload
 };
 
 std::map<Microenvironment *, PluginState> g_state;
@@ -225,6 +226,42 @@ void bfvmb_hip_cell_sources_and_sinks(Microenvironment &M, std::vector<Basic_Age
         return;
     if (bfvmb_simulate_cell_sources_and_sinks(S->ctx) != BFVMB_OK) {
         fail("simulate_cell_sources_and_sinks", S->ctx);
+        return;
+    }
+    S->host_current = false;
+    if (!S->resident)
+        to_host(M, *S);
+}
+
+void bfvmb_hip_bulk_sources_and_sinks(Microenvironment &M, double dt)
+{
+    PluginState *S = state_for(M);
+    if (!S)
+        return;
+    // the rates the bulk_*_function slots give now, voxel by voxel, as the reference
+    // evaluates them on every call (BioFVM_microenvironment.cpp:1122-1128); uploaded when
+    // they differ from the last upload
+    const size_t nv = (size_t)S->nx * S->ny * S->nz, ns = (size_t)S->nsub;
+    std::vector<double> r[3];
+    for (auto &v : r)
+        v.assign(nv * ns, 0.0);
+    for (size_t i = 0; i < nv; ++i) {
+        M.bulk_supply_rate_function(&M, (int)i, &r[0][i * ns]);
+        M.bulk_supply_target_densities_function(&M, (int)i, &r[1][i * ns]);
+        M.bulk_uptake_rate_function(&M, (int)i, &r[2][i * ns]);
+    }
+    if (r[0] != S->bulk[0] || r[1] != S->bulk[1] || r[2] != S->bulk[2]) {
+        if (bfvmb_set_bulk_rates(S->ctx, r[0].data(), r[1].data(), r[2].data()) != BFVMB_OK) {
+            fail("set_bulk_rates", S->ctx);
+            return;
+        }
+        for (int k = 0; k < 3; ++k)
+            S->bulk[k].swap(r[k]);
+    }
+    if (!to_device(M, *S))
+        return;
+    if (bfvmb_simulate_bulk_sources_and_sinks(S->ctx, dt) != BFVMB_OK) {
+        fail("simulate_bulk_sources_and_sinks", S->ctx);
         return;
     }
     S->host_current = false;

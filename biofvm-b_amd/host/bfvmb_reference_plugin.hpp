@@ -22,6 +22,9 @@ void bfvmb_hip_LOD_3D(BioFVM::Microenvironment &M, double dt, int mpi_Size, int 
 void bfvmb_hip_cell_sources_and_sinks(BioFVM::Microenvironment &M, std::vector<BioFVM::Basic_Agent *> &agents,
                                       double dt);
 
+// replaces Microenvironment::simulate_bulk_sources_and_sinks (BioFVM_microenvironment.cpp:1110-1153)
+void bfvmb_hip_bulk_sources_and_sinks(BioFVM::Microenvironment &M, double dt);
+
 void bfvmb_plugin_set_resident(BioFVM::Microenvironment &M, bool resident);
 void bfvmb_plugin_sync(BioFVM::Microenvironment &M);        // device -> p_density_vectors
 void bfvmb_plugin_invalidate(BioFVM::Microenvironment &M);  // host field edited: upload next call

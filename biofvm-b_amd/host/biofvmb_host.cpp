@@ -239,6 +239,39 @@ void Microenvironment::sync_to_host()
     }
 }
 
+void zero_function(Microenvironment *m, int, double *write_destination)
+{
+    for (int i = 0; i < m->number_of_densities(); ++i)
+        write_destination[i] = 0.0;
+}
+
+void Microenvironment::update_rates()
+{
+    ensure_context();
+    const size_t nv = (size_t)number_of_voxels(), ns = (size_t)number_of_densities();
+    std::vector<double> r[3];
+    for (auto &v : r)
+        v.assign(nv * ns, 0.0);
+    for (size_t i = 0; i < nv; ++i) {
+        bulk_supply_rate_function(this, (int)i, &r[0][i * ns]);
+        bulk_supply_target_densities_function(this, (int)i, &r[1][i * ns]);
+        bulk_uptake_rate_function(this, (int)i, &r[2][i * ns]);
+    }
+    if (r[0] == bulk_[0] && r[1] == bulk_[1] && r[2] == bulk_[2])
+        return;
+    check(bfvmb_set_bulk_rates(ctx_, r[0].data(), r[1].data(), r[2].data()), "bfvmb_set_bulk_rates");
+    for (int k = 0; k < 3; ++k)
+        bulk_[k].swap(r[k]);
+}
+
+void Microenvironment::simulate_bulk_sources_and_sinks(double dt)
+{
+    sync_to_device();
+    update_rates();
+    check(bfvmb_simulate_bulk_sources_and_sinks(ctx_, dt), "bfvmb_simulate_bulk_sources_and_sinks");
+    host_stale_ = true;
+}
+
 void Microenvironment::write_to_matlab(const std::string &filename)
 {
     sync_to_device();

@@ -67,6 +67,10 @@ extern std::vector<Basic_Agent *> all_basic_agents;
 Basic_Agent *create_basic_agent();
 void delete_all_basic_agents();
 
+class Microenvironment;
+// Default of the bulk rate slots (BioFVM_microenvironment.cpp:101-108): zeros.
+void zero_function(Microenvironment *m, int voxel_index, double *write_destination);
+
 class Microenvironment {
 public:
     // precision: BFVMB_FP64 (drop-in) or BFVMB_FP32 (FP32 storage, FP64 recurrence)
@@ -110,6 +114,15 @@ public:
     void sync_to_host();
     void sync_to_device();
 
+    // ---- bulk sources and sinks (BioFVM_microenvironment.h:165-167, .cpp:1110-1198): the
+    // slots are evaluated for every voxel and the rates go to the device when they changed;
+    // run() then starts each step with the bulk update too
+    void (*bulk_supply_rate_function)(Microenvironment *, int, double *) = zero_function;
+    void (*bulk_supply_target_densities_function)(Microenvironment *, int, double *) = zero_function;
+    void (*bulk_uptake_rate_function)(Microenvironment *, int, double *) = zero_function;
+    void update_rates();
+    void simulate_bulk_sources_and_sinks(double dt);
+
     // ---- MAT v4 files of the field
     // Microenvironment::write_to_matlab (BioFVM_microenvironment.cpp:1079-1104): written
     // from the device, where the field lives, after any pending host edits are uploaded
@@ -151,6 +164,7 @@ private:
     size_t agents_n_ = 0;
     bool agents_uploaded_ = false;
     unsigned long long agents_epoch_ = 0;
+    std::vector<double> bulk_[3];  // rates at the last upload
     bfvmb_ctx *ctx_ = nullptr;
 };
 

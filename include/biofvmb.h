@@ -131,8 +131,8 @@ int bfvmb_launch_phase(bfvmb_ctx *ctx, int phase, int count);
  * Used by the autotuner; results do not depend on it. */
 int bfvmb_set_launch(bfvmb_ctx *ctx, int sweep, int grid_blocks, int lds_blocks_per_cu, int ring_depth);
 
-/* Autotuner: times the candidate launch shapes of each sweep (LDS wavefronts per CU 1..3;
- * ring depth 16/32/64 for y/z) over `reps` launches with HIP events and keeps the fastest.
+/* Autotuner: times the candidate launch shapes of each sweep (LDS wavefronts per CU 2, 3,
+ * 4, 6, 8; ring depth 32/64 for y/z) over `reps` launches with HIP events and keeps the fastest.
  * The field is saved and restored, so tuning does not advance the simulation.  `chosen`
  * (may be NULL) receives 3 x {wavefronts per CU, ring, microseconds per launch}. */
 int bfvmb_autotune(bfvmb_ctx *ctx, int reps, int *chosen);

@@ -53,6 +53,12 @@ class Case:
         return np.array([self.x0, self.x0, self.x0, self.x0 + self.nx * self.dx,
                          self.x0 + self.ny * self.dx, self.x0 + self.nz * self.dx])
 
+    def write_bulk(self, path: str) -> None:
+        """Bulk supply, target, uptake rates as the drivers' --bulk file."""
+        with open(path, "wb") as f:
+            for a in self.bulk:
+                np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+
     def write(self, path: str) -> None:
         n = self.nagents
         with open(path, "wb") as f:
@@ -114,9 +120,7 @@ def run_reference(case: Case, workdir: str, solver: str = "lod", granularity: in
         args += ["--grad", gpath]
     if case.bulk is not None:
         bpath = os.path.join(workdir, "bulk.bin")
-        with open(bpath, "wb") as f:
-            for a in case.bulk:
-                np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+        case.write_bulk(bpath)
         args += ["--bulk", bpath]
     proc = subprocess.run(args, env=env, capture_output=True, text=True, check=True)
     out = read_output(opath, case)

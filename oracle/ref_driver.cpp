@@ -215,13 +215,10 @@ int main(int argc, char **argv)
         if (!bulk_path.empty()) {
 #ifdef BFVMB_PLUGIN
             if (hip)
-                bfvmb_plugin_sync(M);  // the bulk update runs on the host copy
+                bfvmb_hip_bulk_sources_and_sinks(M, dt);
+            else
 #endif
-            M.simulate_bulk_sources_and_sinks(dt);
-#ifdef BFVMB_PLUGIN
-            if (hip)
-                bfvmb_plugin_invalidate(M);
-#endif
+                M.simulate_bulk_sources_and_sinks(dt);
         }
         if (with_agents) {
 #ifdef BFVMB_PLUGIN

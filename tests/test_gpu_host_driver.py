@@ -52,3 +52,21 @@ def test_host_driver_writes_reference_mat_file(name, mode):
         subprocess.run(args, check=True, timeout=120)
         with open(mpath, "rb") as f:
             assert f.read() == golden.load_mat(name)
+
+
+@pytest.mark.parametrize("mode", ["per_call", "fused"])
+@pytest.mark.parametrize("name", golden.BULK_NAMES)
+def test_host_driver_bulk_matches_reference(name, mode):
+    """The host layer's bulk_*_function slots + simulate_bulk_sources_and_sinks (per call)
+    or update_rates + run (fused) against the reference's own bulk outputs."""
+    case, ref = golden.load(name)
+    print(f"[gpu] host driver bulk {name} {mode}", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        cpath, opath, bpath = (os.path.join(d, f) for f in ("case.bin", "out.bin", "bulk.bin"))
+        case.write(cpath)
+        case.write_bulk(bpath)
+        args = [DRIVER, cpath, opath, "fp64", "--bulk", bpath] + (["--fused"] if mode == "fused" else [])
+        subprocess.run(args, check=True, timeout=120)
+        out = cf.read_output(opath, case)
+    for step, f in ref["fields"].items():
+        assert np.array_equal(out["fields"][step], f), f"{name} step {step}"

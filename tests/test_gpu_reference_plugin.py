@@ -54,3 +54,22 @@ def test_reference_driver_mat_file(name, mode):
         subprocess.run([DRIVER, cpath, opath, mode, "1", "--mat", mpath], check=True, timeout=120, env=env)
         with open(mpath, "rb") as f:
             assert f.read() == golden.load_mat(name)
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="reference build with the plugin (make -C oracle ref-hip) absent")
+@pytest.mark.parametrize("mode", ["hip", "hipres"])
+@pytest.mark.parametrize("name", golden.BULK_NAMES)
+def test_reference_driver_bulk(name, mode):
+    """bfvmb_hip_bulk_sources_and_sinks in place of simulate_bulk_sources_and_sinks in the
+    reference's own driver, with the rates coming from its bulk_*_function slots."""
+    case, ref = golden.load(name)
+    print(f"[gpu] reference driver + plugin bulk {name} {mode}", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        cpath, opath, bpath = (os.path.join(d, f) for f in ("case.bin", "out.bin", "bulk.bin"))
+        case.write(cpath)
+        case.write_bulk(bpath)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([DRIVER, cpath, opath, mode, "1", "--bulk", bpath], check=True, timeout=120, env=env)
+        out = cf.read_output(opath, case)
+    for step, f in ref["fields"].items():
+        assert np.array_equal(out["fields"][step], f), f"{name} step {step}"
