@@ -45,6 +45,16 @@ namespace {
 typedef const __attribute__((address_space(4))) double *ctab;
 __device__ __forceinline__ ctab as_ctab(const double *p) { return (ctab)p; }
 
+// Field rows are accessed through the global address space: a generic (flat) access also
+// counts on the LDS counter, so every wait for the line store would drain the rows in flight.
+template <class T>
+using gmem = __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ gmem<T> as_global(T *p)
+{
+    return (gmem<T>)p;
+}
+
 // a/d correctly rounded from r = RN(1/d): q = a*r, two corrections (Markstein).  Valid for
 // |a| >= 2^-960 (no underflow in the corrections); callers flag smaller numerators.
 __device__ __forceinline__ double div_corr(double a, double d, double r)
@@ -177,13 +187,13 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
         const int x = (int)(line - (long long)o * nx);
         const size_t base = (size_t)s * nvox + (AXIS == 1 ? (size_t)o * nxy + x : (size_t)line);
         const int on = AXIS == 1 ? nz : ny;
-        const bool act = D.dact[s] != 0;
-        const T dv = D.dval[s];
-        const T *src = D.field[parity] + base;
-        T *dst = D.field[parity ^ 1] + base;
+        const bool act = __builtin_amdgcn_readfirstlane((int)D.dact[s]) != 0;
+        const T dv = ((const __attribute__((address_space(4))) T *)D.dval)[s];
+        const gmem<const T> src = as_global<const T>(D.field[parity] + base);
+        const gmem<T> dst = as_global<T>(D.field[parity ^ 1] + base);
         // a line on an active face is computed along but stored as the Dirichlet value
         const bool face = act && (x == 0 || x == nx - 1 || o == 0 || o == on - 1);
-        const double c1 = D.c1[s];
+        const double c1 = as_ctab(D.c1)[s];
 
         // ---- forward elimination; a ring of P rows is in flight.  The head [0, lo) is only
         // swept here (its results are recomputed segment by segment in the backward pass,
@@ -375,38 +385,60 @@ __global__ __launch_bounds__(64) void k_sweep_x(const DomainDesc<T> *__restrict_
         const int rows_here = min(kWave, nrows - r0);
         const int my_row = r0 + min(lane, rows_here - 1);
         const int yy = my_row % ny, zz = my_row / ny;
-        const bool act = D.dact[s] != 0;
-        const T dv = D.dval[s];
+        // per-substrate constants as scalar (wave-uniform) values
+        const bool act = __builtin_amdgcn_readfirstlane((int)D.dact[s]) != 0;
+        const T dv = ((const __attribute__((address_space(4))) T *)D.dval)[s];
         const bool whole = act && (yy == 0 || yy == ny - 1 || zz == 0 || zz == nz - 1);
-        const T *src = D.field[parity] + (size_t)s * nvox + (size_t)r0 * nx;
-        T *dst = D.field[parity ^ 1] + (size_t)s * nvox + (size_t)r0 * nx;
-        const double c1 = D.c1[s];
+        const gmem<const T> src = as_global<const T>(D.field[parity] + (size_t)s * nvox + (size_t)r0 * nx);
+        const gmem<T> dst = as_global<T>(D.field[parity ^ 1] + (size_t)s * nvox + (size_t)r0 * nx);
+        const double c1 = as_ctab(D.c1)[s];
         const ctab den = as_ctab(D.tab[0].den + (size_t)s * nx);
         const ctab cc = as_ctab(D.tab[0].cc + (size_t)s * nx);
         const ctab rden = as_ctab(D.tab[0].rden + (size_t)s * nx);
 
         // chunk c of this tile <-> registers: instruction i covers rows RPI*i .. RPI*i+3,
-        // columns c*W .. c*W+15; out-of-range lanes re-read an in-range element
-        auto load_chunk = [&](const T *from, int c, T *r) {
-            const int col = min(c * W + lcol, nx - 1);
+        // columns c*W .. c*W+15; out-of-range lanes re-read an in-range element.  Whole
+        // chunks (all 64 rows, all 16 columns) address with a uniform base and one lane offset.
+        const bool full_rows = rows_here == kWave;
+        const unsigned lane_off = (unsigned)(lrow * nx + lcol);
+        auto load_chunk = [&](gmem<const T> from, int c, T *r) {
+            if (full_rows && c * W + W <= nx) {
+                const gmem<const T> b = from + c * W;
 #pragma unroll
-            for (int i = 0; i < NI; ++i) {
-                const int rr = min(RPI * i + lrow, rows_here - 1);
-                r[i] = from[(size_t)rr * nx + col];
+                for (int i = 0; i < NI; ++i)
+                    r[i] = b[(size_t)(RPI * i) * nx + lane_off];
+            } else {
+                const int col = min(c * W + lcol, nx - 1);
+#pragma unroll
+                for (int i = 0; i < NI; ++i) {
+                    const int rr = min(RPI * i + lrow, rows_here - 1);
+                    r[i] = from[(size_t)rr * nx + col];
+                }
             }
         };
-        auto put_chunk = [&](const T *r) {
+        // registers -> staging tile; with Dirichlet active, the x-face columns read as the
+        // Dirichlet value (the overwrite before the x sweep)
+        auto put_chunk = [&](const T *r, int c) {
+            const int col = c * W + lcol;
+            const bool edge = act && (col == 0 || col == nx - 1);
 #pragma unroll
             for (int i = 0; i < NI; ++i)
-                stg[(RPI * i + lrow) * (W + 1) + lcol] = r[i];
+                stg[(RPI * i + lrow) * (W + 1) + lcol] = edge ? dv : r[i];
         };
         auto store_chunk = [&](int c) {  // staging tile -> dst, coalesced
-            const int col = c * W + lcol;
+            if (full_rows && c * W + W <= nx) {
+                const gmem<T> b = dst + c * W;
 #pragma unroll
-            for (int i = 0; i < NI; ++i) {
-                const int rr = RPI * i + lrow;
-                if (rr < rows_here && col < nx)
-                    dst[(size_t)rr * nx + col] = stg[rr * (W + 1) + lcol];
+                for (int i = 0; i < NI; ++i)
+                    b[(size_t)(RPI * i) * nx + lane_off] = stg[(RPI * i + lrow) * (W + 1) + lcol];
+            } else {
+                const int col = c * W + lcol;
+#pragma unroll
+                for (int i = 0; i < NI; ++i) {
+                    const int rr = RPI * i + lrow;
+                    if (rr < rows_here && col < nx)
+                        dst[(size_t)rr * nx + col] = stg[rr * (W + 1) + lcol];
+                }
             }
         };
 
@@ -424,7 +456,7 @@ __global__ __launch_bounds__(64) void k_sweep_x(const DomainDesc<T> *__restrict_
         auto fwd_chunk = [&](int c, T *slot, int next, int base, auto store_tag, auto exact_tag, double &amin) {
             constexpr bool STORE = decltype(store_tag)::value;
             constexpr bool EXACT = decltype(exact_tag)::value;
-            put_chunk(slot);
+            put_chunk(slot, c);
             if (next >= 0)  // next < 0: nothing left to prefetch into this slot
                 load_chunk(src, next, slot);
             __syncthreads();
@@ -435,8 +467,7 @@ __global__ __launch_bounds__(64) void k_sweep_x(const DomainDesc<T> *__restrict_
                 const int kk = min(c * W + j, nx - 1);
                 dd[j] = den[kk];
                 rd[j] = rden[kk];
-                const double u = (double)stg[lane * (W + 1) + j];
-                v[j] = (act && (kk == 0 || kk == nx - 1)) ? (double)dv : u;
+                v[j] = (double)stg[lane * (W + 1) + j];
             }
             T *tgt = ls + lane * Lp + (c - base) * W;
             auto step = [&](int j) {
