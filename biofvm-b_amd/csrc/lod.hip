@@ -65,19 +65,21 @@ __device__ __forceinline__ double div_corr(double a, double d, double r)
     e = fma(-q, d, a);
     return fma(e, r, q);
 }
-constexpr double kTiny = 0x1p-960;  // smallest numerator the corrected reciprocal handles
+// Smallest numerator the corrected reciprocal handles, 2^-960, as the upper 32 bits of its
+// pattern (the lower ones are zero): |a| < 2^-960 exactly when hi32(|a|) < kTinyHi.
+constexpr unsigned kTinyHi = (1023u - 960u) << 20;
 
 // One forward step in double: y = fma(c1, prev, v) / den.  With prev = -0.0 the same
 // expression gives the first element's v / den[0] exactly, signed zeros included.
-// FP64 storage: corrected reciprocal, `amin` tracks the smallest |numerator| so that a
-// numerator outside its range can be detected once per line;
+// FP64 storage: corrected reciprocal; `amin` tracks the smallest upper half of |numerator|
+// (two integer operations) so that a numerator outside its range is detected once per line;
 // FP32 storage: the FP64 reciprocal.
 template <class T>
-__device__ __forceinline__ double fwd(double c1, double prev, double v, double den, double rden, double &amin)
+__device__ __forceinline__ double fwd(double c1, double prev, double v, double den, double rden, unsigned &amin)
 {
     const double num = fma(c1, prev, v);
     if constexpr (sizeof(T) == 8) {
-        amin = fmin(amin, fabs(num));
+        amin = min(amin, (unsigned)((unsigned long long)__double_as_longlong(num) >> 32) & 0x7fffffffu);
         return div_corr(num, den, rden);
     } else {
         (void)den;
@@ -152,7 +154,7 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
     int tab_dom = -1, tab_s = -1;
 
     for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const int di = find_domain(prefix, ndom, tile);
+        const int di = __builtin_amdgcn_readfirstlane(find_domain(prefix, ndom, tile));  // uniform
         const DomainDesc<T> &D = descs[di];
         const int nx = D.nx, ny = D.ny, nz = D.nz;
         const size_t nxy = (size_t)nx * ny;
@@ -167,7 +169,8 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
         const size_t stride = AXIS == 1 ? (size_t)nx : nxy;
         const int lo = park_boundary(n, L, P);
         const long long lt = tile - prefix[di];
-        const int s = (int)(lt / tiles_per_s);
+        const int s = __builtin_amdgcn_readfirstlane((int)(lt / tiles_per_s));  // 64-bit division runs on
+                                                                              // the vector unit
         if (di != tab_dom || s != tab_s) {  // stage this substrate's Thomas tables
             __syncthreads();
             const double *gd = D.tab[AXIS].den + (size_t)s * n, *gr = D.tab[AXIS].rden + (size_t)s * n;
@@ -206,7 +209,7 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
         for (int j = 0; j < P; ++j)
             buf[j] = src[(size_t)min(j, n - 1) * stride];
         double y = -0.0;
-        double amin = 1.0;
+        unsigned amin = 0xffffffffu;
         // one full chunk of P rows from k0: prefetches row min(k + P, lim) (PF), stores the
         // results in the line store at row k - base (STORE)
         auto chunk = [&](int k0, int base, int lim, auto store_tag, auto exact_tag, auto pf_tag) {
@@ -263,7 +266,7 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
             // a numerator below 2^-960 somewhere in the tile: redo the forward pass with
             // true divisions (same results everywhere else); the head recomputation below
             // then divides the same way
-            if (__builtin_expect(__any(amin < kTiny), 0)) {
+            if (__builtin_expect(__any(amin < kTinyHi), 0)) {
                 exact = true;
                 y = -0.0;
                 for (int k = 0; k < n; ++k) {
@@ -367,7 +370,7 @@ __global__ __launch_bounds__(64) void k_sweep_x(const DomainDesc<T> *__restrict_
     const long long ntiles = prefix[ndom];
 
     for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int di = find_domain(prefix, ndom, t);
+        const int di = __builtin_amdgcn_readfirstlane(find_domain(prefix, ndom, t));  // uniform
         const DomainDesc<T> &D = descs[di];
         const int nx = D.nx, ny = D.ny, nz = D.nz;
         const size_t nvox = (size_t)nx * ny * nz;
@@ -380,8 +383,9 @@ __global__ __launch_bounds__(64) void k_sweep_x(const DomainDesc<T> *__restrict_
         if (cl < 0)
             cl = 0;
         const long long lt = t - prefix[di];
-        const int s = (int)(lt / tiles_per_s);
-        const int r0 = (int)(lt - (long long)s * tiles_per_s) * kWave;
+        const int s = __builtin_amdgcn_readfirstlane((int)(lt / tiles_per_s));  // 64-bit division runs on
+                                                                              // the vector unit
+        const int r0 = __builtin_amdgcn_readfirstlane((int)(lt - (long long)s * tiles_per_s) * kWave);
         const int rows_here = min(kWave, nrows - r0);
         const int my_row = r0 + min(lane, rows_here - 1);
         const int yy = my_row % ny, zz = my_row / ny;
@@ -453,7 +457,7 @@ __global__ __launch_bounds__(64) void k_sweep_x(const DomainDesc<T> *__restrict_
         double y = -0.0;
         // chunk c: stage the ring slot, refill it with chunk `next`, then W recurrence steps;
         // results to the line store at chunk c - base when STORE
-        auto fwd_chunk = [&](int c, T *slot, int next, int base, auto store_tag, auto exact_tag, double &amin) {
+        auto fwd_chunk = [&](int c, T *slot, int next, int base, auto store_tag, auto exact_tag, unsigned &amin) {
             constexpr bool STORE = decltype(store_tag)::value;
             constexpr bool EXACT = decltype(exact_tag)::value;
             put_chunk(slot, c);
@@ -493,7 +497,7 @@ __global__ __launch_bounds__(64) void k_sweep_x(const DomainDesc<T> *__restrict_
         const std::false_type no_store, inexact;
         const std::true_type store, exact_div;
         auto forward = [&](auto exact_tag) {
-            double amin = 1.0;
+            unsigned amin = 0xffffffffu;
 #pragma unroll
             for (int p = 0; p < PC; ++p)
                 load_chunk(src, min(p, nch - 1), ring[p]);
@@ -518,9 +522,9 @@ __global__ __launch_bounds__(64) void k_sweep_x(const DomainDesc<T> *__restrict_
             return amin;
         };
         bool exact = false;
-        const double amin = forward(inexact);
+        const unsigned amin = forward(inexact);
         if constexpr (sizeof(T) == 8) {
-            if (__builtin_expect(__any(amin < kTiny), 0)) {
+            if (__builtin_expect(__any(amin < kTinyHi), 0)) {
                 exact = true;
                 forward(exact_div);
             }
@@ -578,7 +582,7 @@ __global__ __launch_bounds__(64) void k_sweep_x(const DomainDesc<T> *__restrict_
         for (int cb = cl; cb > 0;) {
             const int ca = seg_begin(cb);
             y = yck;
-            double unused = 1.0;
+            unsigned unused = 0xffffffffu;
             for (int c0 = ca; c0 < cb; c0 += PC) {
 #pragma unroll
                 for (int p = 0; p < PC; ++p) {
