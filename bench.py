@@ -253,6 +253,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and adjoint workloads")
     ap.add_argument("--c5", action="store_true", help="also time the FP32 640^3 x 8 workload")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 1 GiB device copy (reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the LOD path is built and measured on a single MI355X "
@@ -261,6 +263,13 @@ def main():
 
     import torch
     stream = torch.cuda.current_stream().cuda_stream
+    if args.copy_calibration:
+        a = torch.ones(128 * 1024 * 1024, dtype=torch.float64, device="cuda")  # 1 GiB
+        b = torch.empty_like(a)
+        for _ in range(2):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     log("generating inputs ...")
     inputs = make_inputs()
 

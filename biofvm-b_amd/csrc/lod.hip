@@ -122,15 +122,21 @@ __host__ __device__ __forceinline__ int yz_line_store(int n, long long budget)
     return L < 0 ? 0 : (int)L;
 }
 
-// Columns per row kept in the x line store (a multiple of the 16-column chunk, odd pitch),
-// after the 64 x 17 staging tile.
+// x-sweep chunk width: 16 columns (one 128-byte cache line of a row in FP64, half of one
+// in FP32, where 32-column chunks measured slower: twice the registers per chunk).
+template <class T>
+constexpr int x_chunk() { return 16; }
+
+// Columns per row kept in the x line store (a multiple of the chunk width, odd pitch),
+// after the 64 x (W+1) staging tile.
 template <class T>
 __host__ __device__ __forceinline__ int x_line_store(int nx, long long budget)
 {
-    const long long stg = (long long)kWave * 17 * sizeof(T);
+    constexpr int W = x_chunk<T>();
+    const long long stg = (long long)kWave * (W + 1) * sizeof(T);
     long long L = budget > stg ? (budget - stg) / (kWave * (long long)sizeof(T)) - 1 : 0;
-    L = L / 16 * 16;
-    const long long need = (nx + 15) / 16 * 16;
+    L = L / W * W;
+    const long long need = (nx + W - 1) / W * W;
     if (L > need)
         L = need;
     return L < 0 ? 0 : (int)L;
@@ -359,9 +365,9 @@ __global__ __launch_bounds__(64) void k_sweep_x(const DomainDesc<T> *__restrict_
                                                 const long long *__restrict__ prefix, int ndom, int parity,
                                                 int budget)
 {
-    constexpr int W = 16;
-    constexpr int RPI = kWave / W;   // rows per load instruction (4)
-    constexpr int NI = kWave / RPI;  // load instructions per chunk (16)
+    constexpr int W = x_chunk<T>();  // columns per chunk
+    constexpr int RPI = kWave / W;   // rows per load instruction (4 in FP64, 2 in FP32)
+    constexpr int NI = kWave / RPI;  // load instructions per chunk (W)
     extern __shared__ unsigned char smem[];
     T *stg = reinterpret_cast<T *>(smem);  // [64][W+1]
     T *ls = stg + kWave * (W + 1);          // [64][L+1]
@@ -453,6 +459,14 @@ __global__ __launch_bounds__(64) void k_sweep_x(const DomainDesc<T> *__restrict_
         const int nhead = cl > 0 ? (cl + Sc - 1) / Sc : 0;
         const int r0c = cl - (nhead - 1) * Sc;  // end of the first head segment
         const size_t my_off = (size_t)min(lane, rows_here - 1) * nx;  // this lane's row
+        // Checkpoint of the head segment starting at chunk ca, in the output tile where that
+        // segment's back-substitution writes last: in a whole tile the 64 lanes fill the
+        // first RPI rows of chunk ca (contiguous row segments: a single scattered element
+        // per row would make L2 fetch a line for every 8-byte write); otherwise the lane's
+        // own row at the column before the segment.
+        auto ck_off = [&](int ca) -> size_t {
+            return full_rows ? (size_t)(lane / W) * nx + (size_t)ca * W + (lane % W) : my_off + (size_t)ca * W - 1;
+        };
         T ring[PC][NI];
         double y = -0.0;
         // chunk c: stage the ring slot, refill it with chunk `next`, then W recurrence steps;
@@ -513,7 +527,7 @@ __global__ __launch_bounds__(64) void k_sweep_x(const DomainDesc<T> *__restrict_
                         else
                             fwd_chunk(c, ring[p], min(c + PC, nch - 1), cl, store, exact_tag, amin);
                         if (c + 1 == next_ck && next_ck < cl) {  // checkpoint at a segment end
-                            dst[my_off + (size_t)next_ck * W - 1] = (T)y;
+                            dst[ck_off(next_ck)] = (T)y;
                             next_ck += Sc;
                         }
                     }
@@ -570,7 +584,7 @@ __global__ __launch_bounds__(64) void k_sweep_x(const DomainDesc<T> *__restrict_
         auto seg_begin = [&](int cb) { return cb > r0c ? cb - Sc : 0; };
         auto prefetch_segment = [&](int cb) {
             const int ca = seg_begin(cb);
-            yck = ca == 0 ? -0.0 : (double)dst[my_off + (size_t)ca * W - 1];
+            yck = ca == 0 ? -0.0 : (double)dst[ck_off(ca)];
 #pragma unroll
             for (int p = 0; p < PC; ++p)
                 load_chunk(src, min(ca + p, cb - 1), ring[p]);
@@ -619,8 +633,9 @@ Shape shape_of(const Group *g, int axis)
     Shape sh;
     sh.grid = u.grid;
     // defaults measured on MI355X at 400^3 x 4 (with the head recomputation): four sweep
-    // wavefronts share a CU's LDS, and y/z keep 32 rows per lane in flight
-    sh.bpc = u.lds_blocks_per_cu > 0 ? u.lds_blocks_per_cu : 4;
+    // wavefronts share a CU's LDS (six for the FP32 x and z sweeps, whose lines take half
+    // the bytes), and y/z keep 32 rows per lane in flight
+    sh.bpc = u.lds_blocks_per_cu > 0 ? u.lds_blocks_per_cu : (g->elem == 4 && axis != 1 ? 6 : 4);
     sh.ring = u.ring > 0 ? u.ring : 32;
     return sh;
 }
@@ -653,9 +668,10 @@ int launch_sweep_t(Group *g, int axis)
     for (const Domain &d : g->doms) {
         size_t full, floor;
         if (axis == 0) {
-            const size_t stg = (size_t)kWave * 17 * sizeof(T);
-            full = stg + (size_t)kWave * ((d.nx + 15) / 16 * 16 + 1) * sizeof(T);
-            floor = std::min(full, stg + (size_t)kWave * 17 * sizeof(T));
+            constexpr int W = x_chunk<T>();
+            const size_t stg = (size_t)kWave * (W + 1) * sizeof(T);
+            full = stg + (size_t)kWave * ((d.nx + W - 1) / W * W + 1) * sizeof(T);
+            floor = std::min(full, stg + (size_t)kWave * (W + 1) * sizeof(T));
         } else {
             const int n = axis == 1 ? d.ny : d.nz;
             full = (size_t)n * 24 + (size_t)kWave * n * sizeof(T);
@@ -670,7 +686,7 @@ int launch_sweep_t(Group *g, int axis)
     int ring = sh.ring;
     if (axis == 0) {
         for (const Domain &d : g->doms)
-            if (x_line_store<T>(d.nx, (long long)lds) < 16)
+            if (x_line_store<T>(d.nx, (long long)lds) < x_chunk<T>())
                 return set_error(g, BFVMB_ERR_ARG, "sweep: LDS budget too small for the x line store");
     } else {
         // a line longer than its line store is recomputed in segments of whole rings, so the

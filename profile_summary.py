@@ -1,0 +1,70 @@
+"""Summaries of rocprofv3 runs of bench.py for profiles/ (reads the rocpd databases).
+
+    python profile_summary.py trace <run_results.db>              -> kernel table (markdown)
+    python profile_summary.py traffic <fetch.db> <write.db> <out.json>
+        FETCH_SIZE / WRITE_SIZE per dispatch of each sweep kernel, scaled by the ratio of the
+        true bytes of the 1 GiB calibration copy (bench.py --copy-calibration) to what the
+        same counter reads for it; writes {kernel: bytes per launch} for bench.py and prints
+        a markdown table.
+"""
+import json
+import sqlite3
+import sys
+from collections import defaultdict
+
+COPY_BYTES = 128 * 1024 * 1024 * 8  # the calibration copy reads and writes 1 GiB
+
+
+def short(name):
+    return name.replace("void bfvmb::(anonymous namespace)::", "").replace("bfvmb::(anonymous namespace)::", "").split("(")[0]
+
+
+def trace(db):
+    c = sqlite3.connect(db)
+    print("| kernel | calls | total (us) | average (us) | % |")
+    print("|---|---|---|---|---|")
+    for r in c.execute("select name,total_calls,total_duration,average,percentage from top_kernels"):
+        print(f"| `{short(r[0])}` | {r[1]} | {r[2] / 1e3:.1f} | {r[3] / 1e3:.1f} | {r[4]:.1f} |")
+
+
+def counter(db, name):
+    c = sqlite3.connect(db)
+    per = defaultdict(list)
+    for kname, cn, v in c.execute("select name, counter_name, counter_value from pmc_events"):
+        if cn == name:
+            per[kname].append(v * 1024.0)  # KiB -> bytes
+    return per
+
+
+def traffic(fetch_db, write_db, out):
+    f, w = counter(fetch_db, "FETCH_SIZE"), counter(write_db, "WRITE_SIZE")
+    cal = {}
+    for label, d in (("fetch", f), ("write", w)):
+        vals = [max(v) for k, v in d.items() if "copyBuffer" in k]  # torch's device copy
+        big = max(vals) if vals else None
+        cal[label] = COPY_BYTES / big if big else 1.0
+    res, rows = {}, []
+    phase = {"k_sweep_x": "sweep_x", "k_sweep_yz<double, 1": "sweep_y", "k_sweep_yz<double, 2": "sweep_z"}
+    for k in f:
+        s = short(k)
+        if "k_sweep" not in s and "k_agents" not in s:
+            continue
+        fb = sum(f[k]) / len(f[k]) * cal["fetch"]
+        wb = sum(w.get(k, [0.0])) / max(1, len(w.get(k, [0.0]))) * cal["write"]
+        rows.append((s, fb, wb))
+        for pre, ph in phase.items():
+            if s.startswith(pre) and "double" in s:
+                res[ph] = fb + wb
+    json.dump({"bytes_per_launch": res, "calibration": cal}, open(out, "w"), indent=1)
+    print(f"calibration (true bytes / counter bytes of the 1 GiB copy): fetch {cal['fetch']:.3f}, write {cal['write']:.3f}")
+    print("| kernel | fetched (GB/launch) | written (GB/launch) | total (GB) |")
+    print("|---|---|---|---|")
+    for s, fb, wb in sorted(rows):
+        print(f"| `{s}` | {fb / 1e9:.3f} | {wb / 1e9:.3f} | {(fb + wb) / 1e9:.3f} |")
+
+
+if __name__ == "__main__":
+    if sys.argv[1] == "trace":
+        trace(sys.argv[2])
+    else:
+        traffic(sys.argv[2], sys.argv[3], sys.argv[4])
