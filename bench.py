@@ -290,8 +290,8 @@ def main():
     achieved = bytes_per_launch / (f64["phases_ms"][dom] / 1e3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_400cube_fp64.json")
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get(dom)
+    if os.path.exists(tpath):  # HBM bytes per launch from FETCH_SIZE + WRITE_SIZE counters (profiles/)
+        traffic = json.load(open(tpath)).get("bytes_per_launch", {}).get(dom)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "algorithmic_bytes_per_launch": bytes_per_launch}

@@ -24,7 +24,7 @@ def trace(db):
     print("| kernel | calls | total (us) | average (us) | % |")
     print("|---|---|---|---|---|")
     for r in c.execute("select name,total_calls,total_duration,average,percentage from top_kernels"):
-        print(f"| `{short(r[0])}` | {r[1]} | {r[2] / 1e3:.1f} | {r[3] / 1e3:.1f} | {r[4]:.1f} |")
+        print(f"| `{short(r[0])}` | {r[1]} | {r[2]:.1f} | {r[3]:.1f} | {r[4]:.1f} |")
 
 
 def counter(db, name):
