@@ -145,8 +145,11 @@ __host__ __device__ __forceinline__ int x_line_store(int nx, long long budget)
 // ------------------------------------------------------------------------------ y / z
 // Block = one wavefront = 64 lines of one substrate.  LDS: {den, rden}[n], cc[n] of the
 // block's substrate, then the line store [L][64].  P = rows in flight per lane (ring
-// depth); table entries are read in batches of TB per wait.
+// depth); table entries are read in batches of TB per wait.  Before the line store, up to
+// kRegRows forward results per line are kept in registers (the wavefront has a SIMD to
+// itself at the default four per CU, and its register file is larger than its LDS share).
 constexpr int TB = 16;
+constexpr int kRegRows = 128;
 
 template <class T, int AXIS, int P>
 __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict__ descs,
@@ -154,6 +157,8 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
                                                  int budget)
 {
     static_assert(P % TB == 0, "ring depth must be a multiple of the table batch");
+    constexpr int RC = (P == 64 ? kRegRows / 2 : kRegRows) / P;  // rings of rows kept in registers
+                                                                 // (fewer beside a 64-row ring)
     extern __shared__ unsigned char smem[];
     const int lane = threadIdx.x;
     const long long ntiles = prefix[ndom];
@@ -173,7 +178,11 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
         const long long lines_per_s = AXIS == 1 ? (long long)nx * nz : (long long)nxy;
         const long long tiles_per_s = (lines_per_s + kWave - 1) / kWave;
         const size_t stride = AXIS == 1 ? (size_t)nx : nxy;
-        const int lo = park_boundary(n, L, P);
+        // rows [lds_lo, n) go to the line store, [lo, lds_lo) to registers (whole rings of
+        // rows), the head [0, lo) is recomputed
+        const int lds_lo = park_boundary(n, L, P);
+        const int nregc = min(RC, lds_lo / P);
+        const int lo = lds_lo - nregc * P;
         const long long lt = tile - prefix[di];
         const int s = __builtin_amdgcn_readfirstlane((int)(lt / tiles_per_s));  // 64-bit division runs on
                                                                               // the vector unit
@@ -216,10 +225,11 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
             buf[j] = src[(size_t)min(j, n - 1) * stride];
         double y = -0.0;
         unsigned amin = 0xffffffffu;
+        T regs[RC * P];  // register rows [lo, lds_lo)
         // one full chunk of P rows from k0: prefetches row min(k + P, lim) (PF), stores the
-        // results in the line store at row k - base (STORE)
-        auto chunk = [&](int k0, int base, int lim, auto store_tag, auto exact_tag, auto pf_tag) {
-            constexpr bool STORE = decltype(store_tag)::value;
+        // results (STORE: 1 = line store at row k - base, 2 = registers rp[j])
+        auto chunk = [&](int k0, int base, int lim, auto store_tag, auto exact_tag, auto pf_tag, T *rp) {
+            constexpr int STORE = decltype(store_tag)::value;
             constexpr bool EXACT = decltype(exact_tag)::value;
             constexpr bool PF = decltype(pf_tag)::value;
 #pragma unroll
@@ -239,23 +249,32 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
                         y = fma(c1, y, v) / t[jj].x;
                     else
                         y = fwd<T>(c1, y, v, t[jj].x, t[jj].y, amin);
-                    if constexpr (STORE)
+                    if constexpr (STORE == 1)
                         ls[(size_t)(k - base) * kWave + lane] = (T)y;
+                    else if constexpr (STORE == 2)
+                        rp[j] = (T)y;
                 }
             }
         };
-        const std::false_type no_store, inexact, no_pf;
-        const std::true_type store, exact_div, pf;
+        const std::integral_constant<int, 0> no_store;
+        const std::integral_constant<int, 1> store;
+        const std::integral_constant<int, 2> reg_store;
+        const std::false_type inexact, no_pf;
+        const std::true_type exact_div, pf;
         int k0 = 0;
         for (int next_ck = r0; k0 < lo; k0 += P) {  // head: checkpoints only
-            chunk(k0, 0, n - 1, no_store, inexact, pf);
+            chunk(k0, 0, n - 1, no_store, inexact, pf, nullptr);
             if (k0 + P == next_ck && next_ck < lo) {
                 dst[(size_t)(next_ck - 1) * stride] = (T)y;
                 next_ck += S;
             }
         }
-        for (; k0 + P <= n; k0 += P)  // tail: line store
-            chunk(k0, lo, n - 1, store, inexact, pf);
+#pragma unroll
+        for (int q = 0; q < RC; ++q)  // register rows
+            if (q < nregc)
+                chunk(lo + q * P, 0, n - 1, reg_store, inexact, pf, regs + q * P);
+        for (k0 = lds_lo; k0 + P <= n; k0 += P)  // tail: line store
+            chunk(k0, lds_lo, n - 1, store, inexact, pf, nullptr);
         if (k0 < n) {  // the last, partial chunk (uniform per-row conditions)
 #pragma unroll
             for (int j = 0; j < P; ++j) {
@@ -263,7 +282,7 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
                 if (k < n) {
                     const double2 t = tab2[k];
                     y = fwd<T>(c1, y, (double)buf[j], t.x, t.y, amin);
-                    ls[(size_t)(k - lo) * kWave + lane] = (T)y;
+                    ls[(size_t)(k - lds_lo) * kWave + lane] = (T)y;
                 }
             }
         }
@@ -275,12 +294,24 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
             if (__builtin_expect(__any(amin < kTinyHi), 0)) {
                 exact = true;
                 y = -0.0;
-                for (int k = 0; k < n; ++k) {
+                for (int k = 0; k < lo; ++k) {
                     y = fma(c1, y, (double)src[(size_t)k * stride]) / tab2[k].x;
-                    if (k >= lo)
-                        ls[(size_t)(k - lo) * kWave + lane] = (T)y;
-                    else if (k + 1 >= r0 && k + 1 < lo && (k + 1 - r0) % S == 0)
+                    if (k + 1 >= r0 && k + 1 < lo && (k + 1 - r0) % S == 0)
                         dst[(size_t)k * stride] = (T)y;
+                }
+#pragma unroll
+                for (int q = 0; q < RC; ++q)
+                    if (q < nregc) {
+#pragma unroll
+                        for (int j = 0; j < P; ++j) {
+                            const int k = lo + q * P + j;
+                            y = fma(c1, y, (double)src[(size_t)k * stride]) / tab2[k].x;
+                            regs[q * P + j] = (T)y;
+                        }
+                    }
+                for (int k = lds_lo; k < n; ++k) {
+                    y = fma(c1, y, (double)src[(size_t)k * stride]) / tab2[k].x;
+                    ls[(size_t)(k - lds_lo) * kWave + lane] = (T)y;
                 }
             }
         }
@@ -330,7 +361,24 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
         };
         if (lo > 0)
             prefetch_segment(lo);
-        back(n - 2, lo);
+        back(n - 2, lds_lo);
+#pragma unroll
+        for (int q = RC - 1; q >= 0; --q)  // register rows, last first
+            if (q < nregc) {
+#pragma unroll
+                for (int jb = P - TB; jb >= 0; jb -= TB) {
+                    double c[TB];
+#pragma unroll
+                    for (int jj = 0; jj < TB; ++jj)
+                        c[jj] = tabc[lo + q * P + jb + jj];
+#pragma unroll
+                    for (int jj = TB - 1; jj >= 0; --jj) {
+                        const int k = lo + q * P + jb + jj;
+                        xv = fma(-c[jj], xv, (double)regs[q * P + jb + jj]);
+                        dst[(size_t)k * stride] = (face || (act && k == 0)) ? dv : (T)xv;
+                    }
+                }
+            }
         for (int b = lo; b > 0;) {
             const int a = seg_begin(b);
             y = yck;
@@ -338,14 +386,14 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
                 const bool last = kk + P >= b;
                 if (exact) {
                     if (last)
-                        chunk(kk, a, b - 1, store, exact_div, no_pf);
+                        chunk(kk, a, b - 1, store, exact_div, no_pf, nullptr);
                     else
-                        chunk(kk, a, b - 1, store, exact_div, pf);
+                        chunk(kk, a, b - 1, store, exact_div, pf, nullptr);
                 } else {
                     if (last)
-                        chunk(kk, a, b - 1, store, inexact, no_pf);
+                        chunk(kk, a, b - 1, store, inexact, no_pf, nullptr);
                     else
-                        chunk(kk, a, b - 1, store, inexact, pf);
+                        chunk(kk, a, b - 1, store, inexact, pf, nullptr);
                 }
             }
             if (a > 0)
@@ -633,9 +681,9 @@ Shape shape_of(const Group *g, int axis)
     Shape sh;
     sh.grid = u.grid;
     // defaults measured on MI355X at 400^3 x 4 (with the head recomputation): four sweep
-    // wavefronts share a CU's LDS (six for the FP32 x and z sweeps, whose lines take half
-    // the bytes), and y/z keep 32 rows per lane in flight
-    sh.bpc = u.lds_blocks_per_cu > 0 ? u.lds_blocks_per_cu : (g->elem == 4 && axis != 1 ? 6 : 4);
+    // wavefronts share a CU's LDS (six for the FP32 x sweep, whose lines take half the
+    // bytes), and y/z keep 32 rows per lane in flight
+    sh.bpc = u.lds_blocks_per_cu > 0 ? u.lds_blocks_per_cu : (g->elem == 4 && axis == 0 ? 6 : 4);
     sh.ring = u.ring > 0 ? u.ring : 32;
     return sh;
 }
