@@ -367,28 +367,32 @@ int domain_set_agents(Group *g, int i, int n, const int *voxel, const double *te
     Domain &d = g->doms[i];
     const int ns = d.nsub;
     std::vector<int> order;
+    std::vector<long long> off(n);  // device offset z*nx*ny + y*nx + x of each agent's voxel
     order.reserve(n);
+    const long long nyz = (long long)d.ny * d.nz, nxy = (long long)d.nx * d.ny;
     for (int a = 0; a < n; ++a) {
         if (active && !active[a])
             continue;
         if (voxel[a] < 0 || (size_t)voxel[a] >= d.nvox)
             return set_error(g, BFVMB_ERR_ARG, "set_agents: voxel index out of range");
+        const long long v = voxel[a];
+        const long long x = v / nyz, rem = v - x * nyz, y = rem / d.nz, z = rem - y * d.nz;
+        off[a] = z * nxy + y * d.nx + x;
         order.push_back(a);
     }
-    // list order is kept inside a voxel, as the reference's loop applies them
-    std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return voxel[p] < voxel[q]; });
+    // Voxels in device order, so that neighbouring threads of the agent kernel touch
+    // neighbouring field elements; list order is kept inside a voxel, as the reference's loop
+    // applies them.
+    std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return off[p] < off[q]; });
     const int nact = (int)order.size();
     std::vector<int> seg_start;
     std::vector<long long> seg_off;
     std::vector<double> t1((size_t)ns * nact), t2((size_t)ns * nact);
-    const long long nyz = (long long)d.ny * d.nz, nxy = (long long)d.nx * d.ny;
     for (int k = 0; k < nact; ++k) {
         const int a = order[k];
-        if (k == 0 || voxel[a] != voxel[order[k - 1]]) {
-            const long long v = voxel[a];
-            const long long x = v / nyz, rem = v - x * nyz, y = rem / d.nz, z = rem - y * d.nz;
+        if (k == 0 || off[a] != off[order[k - 1]]) {
             seg_start.push_back(k);
-            seg_off.push_back(z * nxy + y * d.nx + x);
+            seg_off.push_back(off[a]);
         }
         for (int s = 0; s < ns; ++s) {
             t1[(size_t)s * nact + k] = temp1[(size_t)a * ns + s];
