@@ -408,8 +408,14 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
 // Block = one wavefront = 64 consecutive rows (y,z) of one substrate.  LDS: staging tile
 // [64][W+1] and line store [64][L+1] (odd pitches keep the transposed accesses
 // conflict-free).  Column chunks are W = 16 wide; PC chunks of loads are kept in flight.
+// FP64: at most 256 registers per lane, so that two wavefronts share a SIMD (the x sweep
+// gains more from a second wavefront hiding its latencies than from a deeper ring, and keeps
+// two chunks in flight for that); FP32 fits two wavefronts with three chunks.
+template <class T>
+constexpr int x_ring() { return sizeof(T) == 8 ? 2 : 3; }
+
 template <class T, int PC>
-__global__ __launch_bounds__(64) void k_sweep_x(const DomainDesc<T> *__restrict__ descs,
+__global__ __launch_bounds__(64, sizeof(T) == 8 ? 2 : 1) void k_sweep_x(const DomainDesc<T> *__restrict__ descs,
                                                 const long long *__restrict__ prefix, int ndom, int parity,
                                                 int budget)
 {
@@ -680,10 +686,10 @@ Shape shape_of(const Group *g, int axis)
     const LaunchShape &u = g->launch[axis];
     Shape sh;
     sh.grid = u.grid;
-    // defaults measured on MI355X at 400^3 x 4 (with the head recomputation): four sweep
-    // wavefronts share a CU's LDS (six for the FP32 x sweep, whose lines take half the
-    // bytes), and y/z keep 32 rows per lane in flight
-    sh.bpc = u.lds_blocks_per_cu > 0 ? u.lds_blocks_per_cu : (g->elem == 4 && axis == 0 ? 6 : 4);
+    // defaults measured on MI355X at 400^3 x 4 (with the head recomputation): four y/z sweep
+    // wavefronts share a CU's LDS, six x sweep wavefronts (two per SIMD on some SIMDs), and
+    // y/z keep 32 rows per lane in flight
+    sh.bpc = u.lds_blocks_per_cu > 0 ? u.lds_blocks_per_cu : (axis == 0 ? 6 : 4);
     sh.ring = u.ring > 0 ? u.ring : 32;
     return sh;
 }
@@ -761,7 +767,7 @@ int launch_sweep_t(Group *g, int axis)
     grid = std::min<long long>(grid, 1LL << 30);
     const DomainDesc<T> *d = static_cast<const DomainDesc<T> *>(g->d_desc);
     if (axis == 0)
-        hipLaunchKernelGGL((k_sweep_x<T, 3>), dim3((unsigned)grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur,
+        hipLaunchKernelGGL((k_sweep_x<T, x_ring<T>()>), dim3((unsigned)grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur,
                            (int)lds);
     else if (axis == 1)
         launch_yz<T, 1>(g, d, pre, ndom, (unsigned)grid, lds, ring);
@@ -806,9 +812,9 @@ int sweep_lds_attributes(size_t lds_bytes)
     if (e == hipSuccess)
         e = yz_attributes<float, 2>(bytes);
     if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_sweep_x<double, 3>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        e = hipFuncSetAttribute((const void *)k_sweep_x<double, x_ring<double>()>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_sweep_x<float, 3>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        e = hipFuncSetAttribute((const void *)k_sweep_x<float, x_ring<float>()>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     return e == hipSuccess ? 0 : (int)e;
 }
 
