@@ -408,19 +408,36 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
 // Block = one wavefront = 64 consecutive rows (y,z) of one substrate.  LDS: staging tile
 // [64][W+1] and line store [64][L+1] (odd pitches keep the transposed accesses
 // conflict-free).  Column chunks are W = 16 wide; PC chunks of loads are kept in flight.
-// FP64: at most 256 registers per lane, so that two wavefronts share a SIMD (the x sweep
-// gains more from a second wavefront hiding its latencies than from a deeper ring, and keeps
-// two chunks in flight for that); FP32 fits two wavefronts with three chunks.
+// Variants (PC chunks in flight, REGC register chunks), chosen per launch by x_variant:
+//  * FP64 (2, 0): at most 256 registers per lane, so that two wavefronts share a SIMD (the x
+//    sweep gains more from a second wavefront hiding its latencies than from a deeper ring;
+//    one wavefront per SIMD with register chunks measured no faster).
+//  * FP32 at up to four wavefronts per CU (1, 17): one wavefront per SIMD, one chunk in
+//    flight, and the forward results of 17 chunks kept in registers between the recomputed
+//    head and the line store (an FP32 chunk takes 16 registers), so that with the line
+//    store's 8 chunks a 400-long line is recomputed nowhere.  Measured at 400^3 x 4: deeper
+//    rings with fewer register chunks were slower at every step (3 and 15: -10%).
+//  * FP32 at more wavefronts per CU (3, 0): two wavefronts per SIMD and no register chunks,
+//    the faster choice for short lines that the line store holds whole.
+struct XVariant {
+    int pc, regc;
+};
 template <class T>
-constexpr int x_ring() { return sizeof(T) == 8 ? 2 : 3; }
+XVariant x_variant(int bpc)
+{
+    if (sizeof(T) == 8)
+        return {2, 0};
+    return bpc <= 4 ? XVariant{1, 17} : XVariant{3, 0};
+}
 
-template <class T, int PC>
-__global__ __launch_bounds__(64, sizeof(T) == 8 ? 2 : 1) void k_sweep_x(const DomainDesc<T> *__restrict__ descs,
+template <class T, int PC, int REGC>
+__global__ __launch_bounds__(64, (sizeof(T) == 8 && REGC == 0) ? 2 : 1) void k_sweep_x(const DomainDesc<T> *__restrict__ descs,
                                                 const long long *__restrict__ prefix, int ndom, int parity,
                                                 int budget)
 {
+    static_assert(REGC % PC == 0, "register chunks come in whole rings");
     constexpr int W = x_chunk<T>();  // columns per chunk
-    constexpr int RPI = kWave / W;   // rows per load instruction (4 in FP64, 2 in FP32)
+    constexpr int RPI = kWave / W;   // rows per load instruction (4)
     constexpr int NI = kWave / RPI;  // load instructions per chunk (W)
     extern __shared__ unsigned char smem[];
     T *stg = reinterpret_cast<T *>(smem);  // [64][W+1]
@@ -439,9 +456,17 @@ __global__ __launch_bounds__(64, sizeof(T) == 8 ? 2 : 1) void k_sweep_x(const Do
         const int L = x_line_store<T>(nx, budget);
         const int Lp = L + 1;
         const int nch = (nx + W - 1) / W;
-        int cl = nch - L / W;  // chunks [cl, nch) live in the line store; L is a multiple of W
+        const int Lc = L / W;  // chunks the line store holds (L is a multiple of W)
+        // chunks [cl, nch) live in the line store, [cr, cl) in registers and the head [0, cr)
+        // is recomputed.  With register chunks, cr and cl are multiples of PC, so that chunk c
+        // always sits in ring slot c % PC (the host guarantees Lc >= PC for a line longer
+        // than the store, which keeps cl < nch).
+        int cl = nch - Lc;
+        if constexpr (REGC > 0)
+            cl = (cl + PC - 1) / PC * PC;
         if (cl < 0)
             cl = 0;
+        const int cr = cl - min(REGC, cl);
         const long long lt = t - prefix[di];
         const int s = __builtin_amdgcn_readfirstlane((int)(lt / tiles_per_s));  // 64-bit division runs on
                                                                               // the vector unit
@@ -506,12 +531,12 @@ __global__ __launch_bounds__(64, sizeof(T) == 8 ? 2 : 1) void k_sweep_x(const Do
             }
         };
 
-        // ---- forward.  Head chunks [0, cl) are swept without storing their results (the
+        // ---- forward.  Head chunks [0, cr) are swept without storing their results (the
         // backward pass recomputes them segment by segment from checkpoints kept at the
-        // segment ends); the tail chunks go to the line store.
-        const int Sc = L / W;  // head segment length in chunks (the host guarantees L >= W)
-        const int nhead = cl > 0 ? (cl + Sc - 1) / Sc : 0;
-        const int r0c = cl - (nhead - 1) * Sc;  // end of the first head segment
+        // segment ends); the chunks after them go to registers and the line store.
+        const int Sc = Lc;  // head segment length in chunks (the host guarantees L >= W)
+        const int nhead = cr > 0 ? (cr + Sc - 1) / Sc : 0;
+        const int r0c = cr - (nhead - 1) * Sc;  // end of the first head segment
         const size_t my_off = (size_t)min(lane, rows_here - 1) * nx;  // this lane's row
         // Checkpoint of the head segment starting at chunk ca, in the output tile where that
         // segment's back-substitution writes last: in a whole tile the 64 lanes fill the
@@ -522,11 +547,13 @@ __global__ __launch_bounds__(64, sizeof(T) == 8 ? 2 : 1) void k_sweep_x(const Do
             return full_rows ? (size_t)(lane / W) * nx + (size_t)ca * W + (lane % W) : my_off + (size_t)ca * W - 1;
         };
         T ring[PC][NI];
+        T regs[REGC > 0 ? REGC * W : 1];  // forward results of chunks [cr, cl)
         double y = -0.0;
         // chunk c: stage the ring slot, refill it with chunk `next`, then W recurrence steps;
-        // results to the line store at chunk c - base when STORE
-        auto fwd_chunk = [&](int c, T *slot, int next, int base, auto store_tag, auto exact_tag, unsigned &amin) {
-            constexpr bool STORE = decltype(store_tag)::value;
+        // results (STORE) 1: to the line store at chunk c - base, 2: to registers rp[j]
+        auto fwd_chunk = [&](int c, T *slot, int next, int base, auto store_tag, auto exact_tag, unsigned &amin,
+                             T *rp) __attribute__((always_inline)) {
+            constexpr int STORE = decltype(store_tag)::value;
             constexpr bool EXACT = decltype(exact_tag)::value;
             put_chunk(slot, c);
             if (next >= 0)  // next < 0: nothing left to prefetch into this slot
@@ -547,10 +574,12 @@ __global__ __launch_bounds__(64, sizeof(T) == 8 ? 2 : 1) void k_sweep_x(const Do
                     y = fma(c1, y, v[j]) / dd[j];
                 else
                     y = fwd<T>(c1, y, v[j], dd[j], rd[j], amin);
-                if constexpr (STORE)
+                if constexpr (STORE == 1)
                     tgt[j] = (T)y;
+                else if constexpr (STORE == 2)
+                    rp[j] = (T)y;
             };
-            if (w == W) {
+            if (STORE == 2 || w == W) {  // register chunks lie before the last chunk: never partial
 #pragma unroll
                 for (int j = 0; j < W; ++j)
                     step(j);
@@ -562,8 +591,11 @@ __global__ __launch_bounds__(64, sizeof(T) == 8 ? 2 : 1) void k_sweep_x(const Do
             }
             __syncthreads();
         };
-        const std::false_type no_store, inexact;
-        const std::true_type store, exact_div;
+        const std::integral_constant<int, 0> no_store;
+        const std::integral_constant<int, 1> store;
+        const std::integral_constant<int, 2> reg_store;
+        const std::false_type inexact;
+        const std::true_type exact_div;
         auto forward = [&](auto exact_tag) {
             unsigned amin = 0xffffffffu;
 #pragma unroll
@@ -571,19 +603,46 @@ __global__ __launch_bounds__(64, sizeof(T) == 8 ? 2 : 1) void k_sweep_x(const Do
                 load_chunk(src, min(p, nch - 1), ring[p]);
             y = -0.0;
             int next_ck = r0c;
-            for (int c0 = 0; c0 < nch; c0 += PC) {
+            if constexpr (REGC == 0) {
+                for (int c0 = 0; c0 < nch; c0 += PC) {
 #pragma unroll
-                for (int p = 0; p < PC; ++p) {
-                    const int c = c0 + p;
-                    if (c < nch) {
-                        if (c < cl)
-                            fwd_chunk(c, ring[p], min(c + PC, nch - 1), 0, no_store, exact_tag, amin);
-                        else
-                            fwd_chunk(c, ring[p], min(c + PC, nch - 1), cl, store, exact_tag, amin);
-                        if (c + 1 == next_ck && next_ck < cl) {  // checkpoint at a segment end
+                    for (int p = 0; p < PC; ++p) {
+                        const int c = c0 + p;
+                        if (c < nch) {
+                            if (c < cl)
+                                fwd_chunk(c, ring[p], min(c + PC, nch - 1), 0, no_store, exact_tag, amin, nullptr);
+                            else
+                                fwd_chunk(c, ring[p], min(c + PC, nch - 1), cl, store, exact_tag, amin, nullptr);
+                            if (c + 1 == next_ck && next_ck < cr) {  // checkpoint at a segment end
+                                dst[ck_off(next_ck)] = (T)y;
+                                next_ck += Sc;
+                            }
+                        }
+                    }
+                }
+            } else {
+                for (int c0 = 0; c0 < cr; c0 += PC) {  // head (cr is a multiple of PC)
+#pragma unroll
+                    for (int p = 0; p < PC; ++p) {
+                        const int c = c0 + p;
+                        fwd_chunk(c, ring[p], min(c + PC, nch - 1), 0, no_store, exact_tag, amin, nullptr);
+                        if (c + 1 == next_ck && next_ck < cr) {  // checkpoint at a segment end
                             dst[ck_off(next_ck)] = (T)y;
                             next_ck += Sc;
                         }
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < REGC; ++q)  // register chunks: chunk cr + q in ring slot q % PC
+                    if (cr + q < cl)
+                        fwd_chunk(cr + q, ring[q % PC], min(cr + q + PC, nch - 1), 0, reg_store, exact_tag, amin,
+                                  regs + q * W);
+                for (int c0 = cl; c0 < nch; c0 += PC) {  // line store (cl is a multiple of PC)
+#pragma unroll
+                    for (int p = 0; p < PC; ++p) {
+                        const int c = c0 + p;
+                        if (c < nch)
+                            fwd_chunk(c, ring[p], min(c + PC, nch - 1), cl, store, exact_tag, amin, nullptr);
                     }
                 }
             }
@@ -600,16 +659,20 @@ __global__ __launch_bounds__(64, sizeof(T) == 8 ? 2 : 1) void k_sweep_x(const Do
             (void)amin;
         }
 
-        // ---- backward from the line store: chunk c's forward results sit at c - base
+        // ---- backward: chunk c's forward results sit in the line store at c - base, or in
+        // the registers rp (FROM_REGS)
         double xv = y;
-        auto back_chunk = [&](int c, int base) {
+        auto back_chunk = [&](int c, int base, auto regs_tag, const T *rp) __attribute__((always_inline)) {
+            constexpr bool FROM_REGS = decltype(regs_tag)::value;
             const int w = min(W, nx - c * W);
-            const T *from = ls + lane * Lp + (c - base) * W;
             double cw[W], v[W];
 #pragma unroll
             for (int j = 0; j < W; ++j) {
                 cw[j] = cc[min(c * W + j, nx - 1)];
-                v[j] = (double)from[j];
+                if constexpr (FROM_REGS)
+                    v[j] = (double)rp[j];
+                else
+                    v[j] = (double)ls[lane * Lp + (c - base) * W + j];
             }
             auto bstep = [&](int j) {
                 const int k = c * W + j;
@@ -617,7 +680,7 @@ __global__ __launch_bounds__(64, sizeof(T) == 8 ? 2 : 1) void k_sweep_x(const Do
                     xv = fma(-cw[j], xv, v[j]);
                 stg[lane * (W + 1) + j] = (whole || (act && (k == 0 || k == nx - 1))) ? dv : (T)xv;
             };
-            if (w == W) {
+            if (FROM_REGS || w == W) {
 #pragma unroll
                 for (int j = W - 1; j >= 0; --j)
                     bstep(j);
@@ -643,11 +706,19 @@ __global__ __launch_bounds__(64, sizeof(T) == 8 ? 2 : 1) void k_sweep_x(const Do
             for (int p = 0; p < PC; ++p)
                 load_chunk(src, min(ca + p, cb - 1), ring[p]);
         };
-        if (cl > 0)
-            prefetch_segment(cl);
+        const std::false_type from_ls;
+        const std::true_type from_regs;
+        if (cr > 0)
+            prefetch_segment(cr);
         for (int c = nch - 1; c >= cl; --c)
-            back_chunk(c, cl);
-        for (int cb = cl; cb > 0;) {
+            back_chunk(c, cl, from_ls, nullptr);
+        if constexpr (REGC > 0) {
+#pragma unroll
+            for (int q = REGC - 1; q >= 0; --q)
+                if (cr + q < cl)
+                    back_chunk(cr + q, 0, from_regs, regs + q * W);
+        }
+        for (int cb = cr; cb > 0;) {
             const int ca = seg_begin(cb);
             y = yck;
             unsigned unused = 0xffffffffu;
@@ -658,16 +729,16 @@ __global__ __launch_bounds__(64, sizeof(T) == 8 ? 2 : 1) void k_sweep_x(const Do
                     if (c < cb) {
                         const int next = c + PC < cb ? c + PC : -1;
                         if (exact)
-                            fwd_chunk(c, ring[p], next, ca, store, exact_div, unused);
+                            fwd_chunk(c, ring[p], next, ca, store, exact_div, unused, nullptr);
                         else
-                            fwd_chunk(c, ring[p], next, ca, store, inexact, unused);
+                            fwd_chunk(c, ring[p], next, ca, store, inexact, unused, nullptr);
                     }
                 }
             }
             if (ca > 0)
                 prefetch_segment(ca);
             for (int c = cb - 1; c >= ca; --c)
-                back_chunk(c, ca);
+                back_chunk(c, ca, from_ls, nullptr);
             cb = ca;
         }
     }
@@ -687,9 +758,10 @@ Shape shape_of(const Group *g, int axis)
     Shape sh;
     sh.grid = u.grid;
     // defaults measured on MI355X at 400^3 x 4 (with the head recomputation): four y/z sweep
-    // wavefronts share a CU's LDS, six x sweep wavefronts (two per SIMD on some SIMDs), and
-    // y/z keep 32 rows per lane in flight
-    sh.bpc = u.lds_blocks_per_cu > 0 ? u.lds_blocks_per_cu : (axis == 0 ? 6 : 4);
+    // wavefronts share a CU's LDS, six FP64 x sweep wavefronts (two per SIMD on some SIMDs),
+    // four FP32 x sweep wavefronts (one per SIMD, holding register chunks), and y/z keep 32
+    // rows per lane in flight
+    sh.bpc = u.lds_blocks_per_cu > 0 ? u.lds_blocks_per_cu : (axis == 0 && g->elem == 8 ? 6 : 4);
     sh.ring = u.ring > 0 ? u.ring : 32;
     return sh;
 }
@@ -707,6 +779,20 @@ void launch_yz(const Group *g, const DomainDesc<T> *d, const long long *pre, int
 }
 
 template <class T>
+void launch_x(const Group *g, const DomainDesc<T> *d, const long long *pre, int ndom, unsigned grid, size_t lds,
+              XVariant xv)
+{
+    if constexpr (sizeof(T) == 8) {
+        (void)xv;
+        hipLaunchKernelGGL((k_sweep_x<T, 2, 0>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+    } else if (xv.regc > 0) {
+        hipLaunchKernelGGL((k_sweep_x<T, 1, 17>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+    } else {
+        hipLaunchKernelGGL((k_sweep_x<T, 3, 0>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+    }
+}
+
+template <class T>
 int launch_sweep_t(Group *g, int axis)
 {
     int rc = sync_descriptors(g);
@@ -714,6 +800,7 @@ int launch_sweep_t(Group *g, int axis)
         return rc;
     const int ndom = (int)g->doms.size();
     const Shape sh = shape_of(g, axis);
+    const XVariant xv = x_variant<T>(sh.bpc);
     size_t budget = std::min(kLdsBytes / sh.bpc, g->max_lds_per_block);
     // LDS actually needed by the largest domain, capped by the budget; the budget is raised
     // (fewer wavefronts per CU) where a line store would hold less than one 16-row ring or
@@ -725,7 +812,9 @@ int launch_sweep_t(Group *g, int axis)
             constexpr int W = x_chunk<T>();
             const size_t stg = (size_t)kWave * (W + 1) * sizeof(T);
             full = stg + (size_t)kWave * ((d.nx + W - 1) / W * W + 1) * sizeof(T);
-            floor = std::min(full, stg + (size_t)kWave * (W + 1) * sizeof(T));
+            // one chunk, or one ring of chunks with register chunks
+            const size_t chunks = xv.regc > 0 ? xv.pc : 1;
+            floor = std::min(full, stg + (size_t)kWave * (chunks * W + 1) * sizeof(T));
         } else {
             const int n = axis == 1 ? d.ny : d.nz;
             full = (size_t)n * 24 + (size_t)kWave * n * sizeof(T);
@@ -739,9 +828,12 @@ int launch_sweep_t(Group *g, int axis)
     const size_t lds = std::min(budget, need);
     int ring = sh.ring;
     if (axis == 0) {
-        for (const Domain &d : g->doms)
-            if (x_line_store<T>(d.nx, (long long)lds) < x_chunk<T>())
+        constexpr int W = x_chunk<T>();
+        for (const Domain &d : g->doms) {
+            const int L = x_line_store<T>(d.nx, (long long)lds);
+            if (L < W || (xv.regc > 0 && (d.nx + W - 1) / W > L / W && L / W < xv.pc))
                 return set_error(g, BFVMB_ERR_ARG, "sweep: LDS budget too small for the x line store");
+        }
     } else {
         // a line longer than its line store is recomputed in segments of whole rings, so the
         // store must hold at least one ring: take the deepest ring (<= the requested one)
@@ -767,8 +859,7 @@ int launch_sweep_t(Group *g, int axis)
     grid = std::min<long long>(grid, 1LL << 30);
     const DomainDesc<T> *d = static_cast<const DomainDesc<T> *>(g->d_desc);
     if (axis == 0)
-        hipLaunchKernelGGL((k_sweep_x<T, x_ring<T>()>), dim3((unsigned)grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur,
-                           (int)lds);
+        launch_x<T>(g, d, pre, ndom, (unsigned)grid, lds, xv);
     else if (axis == 1)
         launch_yz<T, 1>(g, d, pre, ndom, (unsigned)grid, lds, ring);
     else
@@ -812,9 +903,11 @@ int sweep_lds_attributes(size_t lds_bytes)
     if (e == hipSuccess)
         e = yz_attributes<float, 2>(bytes);
     if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_sweep_x<double, x_ring<double>()>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        e = hipFuncSetAttribute((const void *)k_sweep_x<double, 2, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_sweep_x<float, x_ring<float>()>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        e = hipFuncSetAttribute((const void *)k_sweep_x<float, 1, 17>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)k_sweep_x<float, 3, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     return e == hipSuccess ? 0 : (int)e;
 }
 
