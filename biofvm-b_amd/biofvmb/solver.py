@@ -30,6 +30,19 @@ def voxel_indices(grid, dx, pos, bbox_min=(0.0, 0.0, 0.0)) -> np.ndarray:
     return out
 
 
+def _positions(pos, n):
+    """(pointer, on_device, keep-alive) of an [n][3] FP64 position array: a torch CUDA tensor
+    is passed as a device pointer, anything else is copied to a contiguous host array."""
+    if hasattr(pos, "is_cuda") and pos.is_cuda:
+        if pos.dtype != __import__("torch").float64 or not pos.is_contiguous() or pos.numel() != 3 * n:
+            raise ValueError("device positions must be a contiguous float64 [n][3] tensor")
+        return ctypes.c_void_p(pos.data_ptr()), 1, pos
+    P = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    if P.shape[0] != n:
+        raise ValueError(f"{P.shape[0]} positions for {n} agents")
+    return ctypes.c_void_p(P.ctypes.data), 0, P
+
+
 class LODSolver:
     def __init__(self, nx: int, ny: int, nz: int, nsub: int, dx: float, precision: str = "fp64",
                  device: int = 0):
@@ -118,6 +131,17 @@ class LODSolver:
         self._nagents = n
         self._check(self._lib.bfvmb_set_agent_rates(self._h, n, L.iptr(v), L.dptr(vol), L.dptr(S), L.dptr(U), L.dptr(T),
                                                     None if a is None else L.bptr(a), float(dt)), "set_agent_rates")
+
+    def update_agent_positions(self, pos, bbox_min=(0.0, 0.0, 0.0)) -> None:
+        """New positions ([n][3], host array or CUDA float64 tensor) of the agents last set:
+        voxels and the grouping by voxel are recomputed on the device; agents outside the
+        bounding box become inactive (bfvmb_update_agent_positions)."""
+        n = getattr(self, "_nagents", 0)
+        ptr, dev, keep = _positions(pos, n)
+        lo = np.ascontiguousarray(bbox_min, dtype=np.float64)
+        self._check(self._lib.bfvmb_update_agent_positions(self._h, n, ptr, L.dptr(lo), dev),
+                    "update_agent_positions")
+        del keep
 
     def compute_gradients(self) -> None:
         """Gradients of the current field into the device buffer (bfvmb_compute_gradients)."""
@@ -255,6 +279,14 @@ class Batch:
         lam = np.ascontiguousarray(decay, dtype=np.float64)
         self._check(self._lib.bfvmb_batch_set_coefficients(self._h, i, L.dptr(D), L.dptr(lam), float(dt)),
                     "set_coefficients")
+
+    def update_agent_positions(self, i, pos, n, bbox_min=(0.0, 0.0, 0.0)):
+        """bfvmb_batch_update_agent_positions for member i (n = length of its agent list)."""
+        ptr, dev, keep = _positions(pos, n)
+        lo = np.ascontiguousarray(bbox_min, dtype=np.float64)
+        self._check(self._lib.bfvmb_batch_update_agent_positions(self._h, i, n, ptr, L.dptr(lo), dev),
+                    "batch_update_agent_positions")
+        del keep
 
     def set_bulk_rates(self, i, supply=None, target=None, uptake=None):
         keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(self.shapes[i])

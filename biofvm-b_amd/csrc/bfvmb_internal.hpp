@@ -77,6 +77,7 @@ struct Domain {
     double *t1 = nullptr, *t2 = nullptr;
     std::vector<int> agent_order;  // sorted position -> index in the caller's agent list
     int agents_given = 0;          // length of the caller's agent list
+    struct AgentRemap *remap = nullptr;  // device regrouping state of moving agents (agents.hip)
     double *bulk = nullptr;        // bulk sources/sinks: T*S then U+S, each [nsub][nz][ny][nx]
     void *grad = nullptr;          // gradients [nsub*3 + axis][nz][ny][nx], field precision
     bool grad_valid = false;       // grad holds the gradients of some field
@@ -146,6 +147,10 @@ int domain_download(Group *g, int i, double *rho);
 int domain_set_agents(Group *g, int i, int n, const int *voxel, const double *temp1, const double *temp2,
                       const uint8_t *active);
 int domain_set_bulk(Group *g, int i, const double *supply, const double *target, const double *uptake);
+// moving agents (agents.hip): new positions of the agents that were set, voxels and
+// regrouping on the device; free_remap releases that state
+int domain_update_agent_positions(Group *g, int i, int n, const double *pos, const double *bbox_min, int on_device);
+void free_remap(Domain &d);
 int group_bulk(Group *g, double dt);
 int group_steps(Group *g, int nsteps, bool with_agents, bool use_graph);
 int group_set_stream(Group *g, void *stream);

@@ -76,6 +76,7 @@ int upload_dirichlet(Group *g, Domain &d)
 
 void free_agents(Domain &d)
 {
+    free_remap(d);
     (void)hipFree(d.seg_start);
     (void)hipFree(d.seg_off);
     (void)hipFree(d.t1);

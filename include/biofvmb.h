@@ -88,6 +88,18 @@ int bfvmb_set_agent_rates(bfvmb_ctx *ctx, int n, const int *voxel, const double 
 int bfvmb_voxel_indices(int nx, int ny, int nz, double dx, const double *bbox_min, int n,
                         const double *pos, int *voxel);
 
+/* Moving agents: new positions ([n][3], micrometres; grid lower corner bbox_min, NULL = the
+ * origin) of the n agents last given to bfvmb_set_agents / bfvmb_set_agent_rates, as
+ * Basic_Agent::update_voxel_index computes their voxels (BioFVM_basic_agent.cpp:137-153 with
+ * BioFVM_parallel.cpp:418-522 at world size 1; the same arithmetic as bfvmb_voxel_indices).
+ * The mapping, the stable regrouping by voxel and the reordering of temp1/temp2 run on the
+ * device; the rates are kept.  An agent whose position lies outside the bounding box is
+ * deactivated, as the reference does, and stays inactive until the agents are set again.
+ * pos_on_device != 0: pos is a device pointer, read in stream order.  Blocks until the new
+ * grouping is in place (one small read-back). */
+int bfvmb_update_agent_positions(bfvmb_ctx *ctx, int n, const double *pos, const double *bbox_min,
+                                 int pos_on_device);
+
 /* Microenvironment::simulate_cell_sources_and_sinks (BioFVM_microenvironment.cpp:1155-1169). */
 int bfvmb_simulate_cell_sources_and_sinks(bfvmb_ctx *ctx);
 
@@ -211,6 +223,9 @@ int bfvmb_batch_upload_density(bfvmb_batch *b, int i, const double *rho_voxel_ma
 int bfvmb_batch_download_density(bfvmb_batch *b, int i, double *rho_voxel_major);
 int bfvmb_batch_set_agents(bfvmb_batch *b, int i, int n, const int *voxel, const double *temp1,
                            const double *temp2, const uint8_t *active);
+/* bfvmb_update_agent_positions for member i. */
+int bfvmb_batch_update_agent_positions(bfvmb_batch *b, int i, int n, const double *pos,
+                                       const double *bbox_min, int pos_on_device);
 /* bfvmb_set_bulk_rates for member i. */
 int bfvmb_batch_set_bulk_rates(bfvmb_batch *b, int i, const double *supply, const double *target,
                                const double *uptake);

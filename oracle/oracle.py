@@ -119,6 +119,24 @@ def voxel_indices(case):
     return out
 
 
+def update_voxel_indices(case, pos, active):
+    """Basic_Agent::update_voxel_index (BioFVM_basic_agent.cpp:137-153) for agents at new
+    positions ``pos`` ([n][3], lower grid corner case.x0): a position outside the bounding
+    box deactivates the agent (voxel -1); otherwise the nearest voxel, clamped into the grid
+    (BioFVM_parallel.cpp:418-522).  An inactive agent stays inactive.  Returns (voxel, active)."""
+    bbox, pbbox = _d(case.bbox)
+    P = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    voxel = np.empty(P.shape[0], dtype=np.int32)
+    act = np.array(active, dtype=np.uint8, copy=True).reshape(-1)
+    for a in range(P.shape[0]):
+        p, pp = _d(P[a])
+        v = lib().orc_voxel_index(pp, pbbox, case.dx, case.dx, case.dx, case.nx, case.ny, case.nz)
+        if v < 0:
+            act[a] = 0
+        voxel[a] = v
+    return voxel, act
+
+
 def agent_tables(case):
     voxel = voxel_indices(case)
     dV = (case.dx * case.dx) * case.dx

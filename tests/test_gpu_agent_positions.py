@@ -1,0 +1,131 @@
+"""Moving agents: bfvmb_update_agent_positions (voxel mapping and regrouping on the device)
+against the oracle's Basic_Agent::update_voxel_index semantics (BioFVM_basic_agent.cpp:137-153):
+FP64 bit for bit, FP32 within 1e-5, host and device position buffers, agents leaving the box
+(deactivated for good), shared voxels (list order), single contexts and batch members."""
+import numpy as np
+import pytest
+
+import casefile as cf
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _case():
+    return cf.make_case(37, 29, 23, 3, seed=21, nagents=3000, nsteps=2, shared_voxels=True)
+
+
+def _moves(case, seed):
+    """Two later position sets: random displacements (some agents leave the box), a group
+    piled into one voxel (list order), and agents coming back into the box."""
+    rng = np.random.default_rng(seed)
+    ext = np.array([case.nx, case.ny, case.nz]) * case.dx
+    p1 = case.pos + rng.normal(0.0, 3.0 * case.dx, case.pos.shape)
+    p1[100:140] = ext * np.array([0.5, 0.25, 0.75])  # shared voxel
+    p1[200:230] = -5.0  # outside: deactivated
+    p2 = p1 + rng.normal(0.0, 2.0 * case.dx, p1.shape)
+    p2[200:215] = ext * 0.5  # back inside: the reference keeps them inactive
+    p2[300:320] = ext  # upper corner, inside
+    return [p1, p2]
+
+
+def _oracle(case, moves, steps):
+    """steps[k] steps (sources/sinks then LOD) after the k-th position set."""
+    voxel, t1, t2 = orc.agent_tables(case)
+    active = np.ones(case.nagents, np.uint8)
+    rho = case.init.copy()
+    for k, n in enumerate(steps):
+        if k > 0:
+            voxel, active = orc.update_voxel_indices(case, moves[k - 1], active)
+        for _ in range(n):
+            orc.sources_sinks(rho, np.maximum(voxel, 0), active, t1, t2)
+            rho = orc.run(case, nsteps=1, rho=rho, with_agents=False)
+    return rho, active
+
+
+def _gpu(case, moves, steps, precision, device_positions=False):
+    from biofvmb import LODSolver
+    s = LODSolver(case.nx, case.ny, case.nz, case.nsub, case.dx, precision)
+    s.set_coefficients(case.D, case.decay, case.dt)
+    s.set_dirichlet(case.dvalue, case.dactive)
+    s.upload(case.init)
+    s.set_agent_rates(orc.voxel_indices(case), case.volume, case.S, case.U, case.T, case.dt)
+    for k, n in enumerate(steps):
+        if k > 0:
+            pos = moves[k - 1]
+            if device_positions:
+                import torch
+                pos = torch.tensor(pos, dtype=torch.float64, device="cuda:0").contiguous()
+                torch.cuda.synchronize()
+            s.update_agent_positions(pos)
+        s.run(n)
+    out = s.download()
+    s.close()
+    return out
+
+
+@pytest.mark.parametrize("device_positions", [False, True])
+def test_moved_agents_fp64_bitwise(device_positions):
+    case = _case()
+    moves = _moves(case, 3)
+    steps = [2, 2, 3]
+    print(f"[gpu] moving agents fp64 device_positions={device_positions}", flush=True)
+    ref, active = _oracle(case, moves, steps)
+    assert 0 < int(active.sum()) < case.nagents
+    got = _gpu(case, moves, steps, "fp64", device_positions)
+    assert np.array_equal(got, ref)
+
+
+def test_moved_agents_fp32_error():
+    case = _case()
+    moves = _moves(case, 4)
+    steps = [1, 2, 2]
+    print("[gpu] moving agents fp32", flush=True)
+    ref, _ = _oracle(case, moves, steps)
+    got = _gpu(case, moves, steps, "fp32")
+    r = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)
+    print(f"fp32 moving agents: max rel {r.max():.3e}")
+    assert r.max() <= 1e-5
+
+
+def test_unmoved_positions_keep_results():
+    """Re-sending the initial positions regroups the same agents the same way."""
+    case = _case()
+    print("[gpu] moving agents: same positions", flush=True)
+    ref = _gpu(case, [case.pos], [2, 2], "fp64")
+    want = orc.run(case, nsteps=4)
+    assert np.array_equal(ref, want)
+
+
+def test_batch_member_moves():
+    from biofvmb import Batch
+    case = _case()
+    moves = _moves(case, 5)
+    other = cf.make_case(11, 9, 13, 2, seed=2, nagents=50, nsteps=1)
+    print("[gpu] moving agents in a batch member", flush=True)
+    ref, _ = _oracle(case, moves, [1, 2])
+    b = Batch([(other.nx, other.ny, other.nz, other.nsub), (case.nx, case.ny, case.nz, case.nsub)], case.dx, "fp64")
+    for i, c in enumerate((other, case)):
+        b.set_coefficients(i, c.D, c.decay, c.dt)
+        b.set_dirichlet(i, c.dvalue, c.dactive)
+        b.upload(i, c.init)
+        voxel, t1, t2 = orc.agent_tables(c)
+        b.set_agents(i, voxel, t1, t2)
+    b.run(1)
+    b.update_agent_positions(1, moves[0], case.nagents)
+    b.run(2)
+    assert np.array_equal(b.download(1), ref)
+    assert np.array_equal(b.download(0), orc.run(other, nsteps=3))
+    b.close()
+
+
+def test_wrong_agent_count_is_rejected():
+    from biofvmb import LODSolver, BioFVMBError
+    case = _case()
+    s = LODSolver(case.nx, case.ny, case.nz, case.nsub, case.dx, "fp64")
+    s.set_coefficients(case.D, case.decay, case.dt)
+    s.set_agent_rates(orc.voxel_indices(case), case.volume, case.S, case.U, case.T, case.dt)
+    s._nagents = case.nagents - 1
+    with pytest.raises(BioFVMBError):
+        s.update_agent_positions(case.pos[:-1])
+    s.close()

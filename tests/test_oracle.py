@@ -172,6 +172,29 @@ def test_voxel_mapping_brute_force():
     assert np.array_equal(got, want)
 
 
+def test_moved_agent_voxels_brute_force():
+    """update_voxel_indices: inside the box the nearest voxel (as for the initial
+    assignment), outside it voxel -1 and the agent inactive; inactive agents stay so."""
+    case = cf.make_case(13, 11, 9, 1, seed=6, nagents=300)
+    rng = np.random.default_rng(1)
+    ext = np.array([case.nx, case.ny, case.nz]) * case.dx
+    pos = rng.uniform(-0.2, 1.2, (300, 3)) * ext
+    pos[:5] = ext  # upper corner: inside, clamped into the last voxel
+    pos[5:8] = 0.0
+    active = np.ones(300, np.uint8)
+    active[10:20] = 0
+    vox, act = orc.update_voxel_indices(case, pos, active)
+    lo, hi = case.bbox[:3], case.bbox[3:]
+    n = np.array([case.nx, case.ny, case.nz])
+    inside = np.all((pos >= lo) & (pos <= hi), axis=1)
+    c = np.clip(np.floor((pos - lo) / case.dx).astype(int), 0, n - 1)
+    want = np.where(inside, (c[:, 0] * n[1] + c[:, 1]) * n[2] + c[:, 2], -1)
+    assert np.array_equal(vox, want)
+    assert np.array_equal(act, (inside & (active == 1)).astype(np.uint8))
+    assert np.all(vox[:5] == case.nx * case.ny * case.nz - 1) and np.all(vox[5:8] == 0)
+    assert 0 < inside.sum() < 300
+
+
 # ---------------------------------------------------------------- bulk sources and sinks
 @pytest.mark.parametrize("name", golden.BULK_NAMES)
 def test_bulk_matches_reference_fixtures(name):
