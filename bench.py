@@ -211,6 +211,38 @@ def bench_adjoint(torch, stream, n=128, nsteps=50, every=10, nagents=10_000, see
     return out
 
 
+def bench_agent_moves(torch, stream, n=400, nagents=1_000_000, reps=10, seed=11):
+    """Moving agents in the drop-in workload's domain (400^3 x 4, 1M agents in a 3 mm
+    sphere): every rep moves all agents (positions already in HBM) and regroups the
+    source/sink list on the device (bfvmb_update_agent_positions)."""
+    from biofvmb import LODSolver
+    rng = np.random.default_rng(seed)
+    ns = 4
+    s = LODSolver(n, n, n, ns, DX, "fp64")
+    s.set_stream(stream)
+    s.set_coefficients([1e5, 1e3, 2.5e3, 5e4], [0.1, 0.01, 0.4, 1e-3], DT)
+    d = rng.normal(size=(nagents, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    pos = n * DX / 2.0 + d * (3000.0 * rng.uniform(0, 1, nagents) ** (1.0 / 3.0))[:, None]
+    vox = s.voxel_indices(pos)
+    s.set_agent_rates(vox, np.full(nagents, 4.0 / 3.0 * np.pi * 125.0), rng.uniform(0, 5, (nagents, ns)),
+                      rng.uniform(0, 20, (nagents, ns)), np.ones((nagents, ns)), DT)
+    moves = [torch.tensor(pos + rng.normal(0.0, DX, pos.shape), dtype=torch.float64, device="cuda:0")
+             for _ in range(reps)]
+    s.update_agent_positions(moves[0])  # set-up of the device regrouping state
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for m in moves:
+        s.update_agent_positions(m)
+    t1 = time.perf_counter()
+    s.close()
+    sec = (t1 - t0) / reps
+    out = {"workload": f"{n}^3x{ns}, {nagents} agents moved by N(0, dx) per axis, positions in HBM",
+           "ms_per_update": sec * 1e3, "value": nagents / sec, "unit": "agents regrouped/s (incl. one read-back)"}
+    log(f"agent moves: {sec * 1e3:.3f} ms per update of {nagents} agents")
+    return out
+
+
 def cpu_baseline():
     """The reference (oracle/_ref) on a bounded sample: 200^3 x 4 with 125k agents (1/8 of
     the domain, same densities), 8 steps; falls back to the C restatement (1 thread)."""
@@ -301,6 +333,7 @@ def main():
         extra["batched"] = bench_batched(torch, stream)
         extra["batched"]["fp32_vs_fp64_dropin"] = extra["batched"]["fp32"]["value"] / f64["value"]
         extra["adjoint"] = bench_adjoint(torch, stream)
+        extra["agent_moves"] = bench_agent_moves(torch, stream)
     if args.c5:
         extra["fp32_640cube_x8"] = bench_c5(torch, stream)
     cpu = None if args.no_cpu else cpu_baseline()

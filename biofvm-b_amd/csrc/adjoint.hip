@@ -246,6 +246,8 @@ int backward_t(Group *g, const double *seed, double *grad_rho0, double *grad_t1,
         return rc;
     g->cur = end;
     g->epoch = tp.epoch;
+    if (!h1.empty() && (rc = domain_agent_order(g, d)))
+        return rc;
     const size_t nag = (size_t)d.agents_given;
     if (grad_t1)
         std::fill(grad_t1, grad_t1 + nag * ns, 0.0);

@@ -34,6 +34,7 @@ struct AgentRemap {
     void *tmp = nullptr;                 // hipcub scratch
     size_t tmp_bytes = 0;
     std::vector<int> host_ids;
+    bool order_stale = false;            // Domain::agent_order not yet read back from vout
 };
 
 void free_remap(Domain &d)
@@ -292,20 +293,33 @@ int domain_update_agent_positions(Group *g, int i, int n, const double *pos, con
     hipLaunchKernelGGL(k_agent_build, grid, block, 0, g->stream, r->kout, r->vout, r->head, r->sid, n0, d.nsub, dead,
                        r->t1o, r->t2o, d.seg_start, d.seg_off, d.t1, d.t2, r->counts);
     int counts[2] = {0, 0};
-    std::vector<int> order(n0);
     if ((rc = hip_check(g, hipGetLastError(), "agent build")) ||
         (rc = hip_check(g, hipMemcpyAsync(counts, r->counts, sizeof(counts), hipMemcpyDeviceToHost, g->stream),
                         "agent counts")) ||
-        (rc = hip_check(g, hipMemcpyAsync(order.data(), r->vout, n0 * sizeof(int), hipMemcpyDeviceToHost, g->stream),
-                        "agent order")) ||
         (rc = hip_check(g, hipStreamSynchronize(g->stream), "agent update")))
         return rc;
-    for (int k = 0; k < n0; ++k)
-        order[k] = r->host_ids[order[k]];
-    d.agent_order.swap(order);
+    r->order_stale = true;  // read back when someone needs it (the adjoint's rate gradients)
     d.nseg = counts[0];
     g->desc_dirty = true;
     g->epoch++;
+    return BFVMB_OK;
+}
+
+int domain_agent_order(Group *g, Domain &d)
+{
+    AgentRemap *r = d.remap;
+    if (!r || !r->order_stale)
+        return BFVMB_OK;
+    std::vector<int> order(r->n0);
+    int rc;
+    if ((rc = hip_check(g, hipMemcpyAsync(order.data(), r->vout, r->n0 * sizeof(int), hipMemcpyDeviceToHost, g->stream),
+                        "agent order")) ||
+        (rc = hip_check(g, hipStreamSynchronize(g->stream), "agent order")))
+        return rc;
+    for (int &k : order)
+        k = r->host_ids[k];
+    d.agent_order.swap(order);
+    r->order_stale = false;
     return BFVMB_OK;
 }
 

@@ -75,7 +75,8 @@ struct Domain {
     int *seg_start = nullptr;
     long long *seg_off = nullptr;
     double *t1 = nullptr, *t2 = nullptr;
-    std::vector<int> agent_order;  // sorted position -> index in the caller's agent list
+    std::vector<int> agent_order;  // sorted position -> index in the caller's agent list (after
+                                   // moves: current once domain_agent_order has run)
     int agents_given = 0;          // length of the caller's agent list
     struct AgentRemap *remap = nullptr;  // device regrouping state of moving agents (agents.hip)
     double *bulk = nullptr;        // bulk sources/sinks: T*S then U+S, each [nsub][nz][ny][nx]
@@ -150,6 +151,7 @@ int domain_set_bulk(Group *g, int i, const double *supply, const double *target,
 // moving agents (agents.hip): new positions of the agents that were set, voxels and
 // regrouping on the device; free_remap releases that state
 int domain_update_agent_positions(Group *g, int i, int n, const double *pos, const double *bbox_min, int on_device);
+int domain_agent_order(Group *g, Domain &d);  // brings Domain::agent_order up to date
 void free_remap(Domain &d);
 int group_bulk(Group *g, double dt);
 int group_steps(Group *g, int nsteps, bool with_agents, bool use_graph);

@@ -129,3 +129,33 @@ def test_wrong_agent_count_is_rejected():
     with pytest.raises(BioFVMBError):
         s.update_agent_positions(case.pos[:-1])
     s.close()
+
+
+def test_adjoint_after_moves_matches_fresh_agent_list():
+    """Gradients after a device regrouping equal those of a context given the moved agents'
+    voxels directly (agents that left the box: inactive, zero rate gradients)."""
+    from biofvmb import LODSolver
+    case = cf.make_case(17, 13, 11, 2, seed=8, nagents=300, nsteps=4, shared_voxels=True)
+    moves = _moves(case, 6)
+    voxel, t1, t2 = orc.agent_tables(case)
+    v1, a1 = orc.update_voxel_indices(case, moves[0], np.ones(case.nagents, np.uint8))
+    seed = np.random.default_rng(3).normal(size=case.init.shape)
+    print("[gpu] adjoint after agent moves", flush=True)
+    out = []
+    for moved in (True, False):
+        s = LODSolver(case.nx, case.ny, case.nz, case.nsub, case.dx, "fp64")
+        s.set_coefficients(case.D, case.decay, case.dt)
+        s.set_dirichlet(case.dvalue, case.dactive)
+        s.upload(case.init)
+        if moved:
+            s.set_agents(voxel, t1, t2)
+            s.update_agent_positions(moves[0])
+        else:
+            s.set_agents(np.maximum(v1, 0), t1, t2, a1)
+        s.adjoint_forward(4, 2)
+        out.append(s.adjoint_backward(seed))
+        s.close()
+    for got, want in zip(out[0], out[1]):
+        assert np.array_equal(got, want)
+    dead = a1 == 0
+    assert dead.any() and np.all(out[0][1][dead] == 0.0) and np.any(out[0][1][~dead] != 0.0)
