@@ -89,6 +89,14 @@ def load(path: str = LIB_PATH):
         if not os.path.exists(path):
             raise BioFVMBError(f"{path} is missing: build it with `make -C biofvm-b_amd` "
                                "or __graft_entry__.build()")
+        # torch ships its own HIP runtime under the same soname (libamdhip64.so.7) as the one
+        # the library links: whichever is loaded first serves the process, and torch finds no
+        # GPU on the other one.  Load torch's first, so that torch tensors and streams keep
+        # working next to the library.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = _c.CDLL(path)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)

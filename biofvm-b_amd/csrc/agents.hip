@@ -25,7 +25,8 @@ struct AgentRemap {
     int n0 = 0;                          // agents active when the list was set
     int *ids = nullptr;                  // [n0] caller index of compact agent i (ascending)
     uint8_t *alive = nullptr;            // [n0] still active
-    double *t1o = nullptr, *t2o = nullptr;  // [nsub][n0] temp1 / temp2 in compact order
+    double *tab = nullptr;               // [n0][nsub][2] temp1, temp2 in compact order (agent-major:
+                                         // the regrouping gathers whole agents)
     double *pos = nullptr;               // [given][3] staging of host positions
     unsigned *kin = nullptr, *kout = nullptr;  // [n0] sort keys (device offset, nvox = dead)
     int *vin = nullptr, *vout = nullptr;       // [n0] compact indices
@@ -42,7 +43,7 @@ void free_remap(Domain &d)
     AgentRemap *r = d.remap;
     if (!r)
         return;
-    for (void *p : {(void *)r->ids, (void *)r->alive, (void *)r->t1o, (void *)r->t2o, (void *)r->pos, (void *)r->kin,
+    for (void *p : {(void *)r->ids, (void *)r->alive, (void *)r->tab, (void *)r->pos, (void *)r->kin,
                     (void *)r->kout, (void *)r->vin, (void *)r->vout, (void *)r->head, (void *)r->sid,
                     (void *)r->counts, r->tmp})
         (void)hipFree(p);
@@ -112,7 +113,7 @@ __global__ __launch_bounds__(256) void k_agent_heads(const unsigned *__restrict_
 __global__ __launch_bounds__(256) void k_agent_build(const unsigned *__restrict__ key, const int *__restrict__ val,
                                                      const int *__restrict__ head, const int *__restrict__ sid,
                                                      int n0, int nsub, unsigned dead,
-                                                     const double *__restrict__ t1o, const double *__restrict__ t2o,
+                                                     const double *__restrict__ tab,
                                                      int *__restrict__ seg_start, long long *__restrict__ seg_off,
                                                      double *__restrict__ t1, double *__restrict__ t2,
                                                      int *__restrict__ counts)
@@ -120,10 +121,10 @@ __global__ __launch_bounds__(256) void k_agent_build(const unsigned *__restrict_
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n0)
         return;
-    const int a = val[k];
+    const double *ta = tab + (size_t)val[k] * nsub * 2;
     for (int s = 0; s < nsub; ++s) {
-        t1[(size_t)s * n0 + k] = t1o[(size_t)s * n0 + a];
-        t2[(size_t)s * n0 + k] = t2o[(size_t)s * n0 + a];
+        t1[(size_t)s * n0 + k] = ta[2 * s];
+        t2[(size_t)s * n0 + k] = ta[2 * s + 1];
     }
     if (key[k] == dead)
         return;
@@ -138,17 +139,18 @@ __global__ __launch_bounds__(256) void k_agent_build(const unsigned *__restrict_
     }
 }
 
-// compact-order tables from the tables set_agents sorted: t[s][cidx[k]] = sorted[s][k]
+// compact-order tables from the tables set_agents sorted: tab[cidx[k]][s] = sorted[s][k]
 __global__ __launch_bounds__(256) void k_agent_unsort(const double *__restrict__ t1, const double *__restrict__ t2,
                                                       const int *__restrict__ cidx, int n0, int nsub,
-                                                      double *__restrict__ t1o, double *__restrict__ t2o)
+                                                      double *__restrict__ tab)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n0)
         return;
+    double *ta = tab + (size_t)cidx[k] * nsub * 2;
     for (int s = 0; s < nsub; ++s) {
-        t1o[(size_t)s * n0 + cidx[k]] = t1[(size_t)s * n0 + k];
-        t2o[(size_t)s * n0 + cidx[k]] = t2[(size_t)s * n0 + k];
+        ta[2 * s] = t1[(size_t)s * n0 + k];
+        ta[2 * s + 1] = t2[(size_t)s * n0 + k];
     }
 }
 
@@ -182,8 +184,7 @@ int remap_setup(Group *g, Domain &d)
     const size_t tab = (size_t)ns * n0 * sizeof(double);
     if ((rc = hip_check(g, hipMalloc((void **)&r->ids, n0 * sizeof(int)), "agent ids")) ||
         (rc = hip_check(g, hipMalloc((void **)&r->alive, n0), "agent flags")) ||
-        (rc = hip_check(g, hipMalloc((void **)&r->t1o, tab), "agent temp1")) ||
-        (rc = hip_check(g, hipMalloc((void **)&r->t2o, tab), "agent temp2")) ||
+        (rc = hip_check(g, hipMalloc((void **)&r->tab, 2 * tab), "agent tables")) ||
         (rc = hip_check(g, hipMalloc((void **)&r->kin, n0 * sizeof(unsigned)), "agent keys")) ||
         (rc = hip_check(g, hipMalloc((void **)&r->kout, n0 * sizeof(unsigned)), "agent keys")) ||
         (rc = hip_check(g, hipMalloc((void **)&r->vin, n0 * sizeof(int)), "agent order")) ||
@@ -219,7 +220,7 @@ int remap_setup(Group *g, Domain &d)
                         "agent ranks")))
         goto fail;
     hipLaunchKernelGGL(k_agent_unsort, dim3((n0 + 255) / 256), dim3(256), 0, g->stream, d.t1, d.t2, d_cidx, n0, ns,
-                       r->t1o, r->t2o);
+                       r->tab);
     if ((rc = hip_check(g, hipGetLastError(), "agent tables")) ||
         (rc = hip_check(g, hipStreamSynchronize(g->stream), "agent set-up")))
         goto fail;
@@ -291,7 +292,7 @@ int domain_update_agent_positions(Group *g, int i, int n, const double *pos, con
         (rc = hip_check(g, hipMemsetAsync(d.seg_start, 0, sizeof(int), g->stream), "agent segments")))
         return rc;
     hipLaunchKernelGGL(k_agent_build, grid, block, 0, g->stream, r->kout, r->vout, r->head, r->sid, n0, d.nsub, dead,
-                       r->t1o, r->t2o, d.seg_start, d.seg_off, d.t1, d.t2, r->counts);
+                       r->tab, d.seg_start, d.seg_off, d.t1, d.t2, r->counts);
     int counts[2] = {0, 0};
     if ((rc = hip_check(g, hipGetLastError(), "agent build")) ||
         (rc = hip_check(g, hipMemcpyAsync(counts, r->counts, sizeof(counts), hipMemcpyDeviceToHost, g->stream),
