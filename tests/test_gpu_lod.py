@@ -141,3 +141,49 @@ def test_launch_shapes_do_not_change_results(name, launch):
             assert np.array_equal(got, ref)
         else:
             assert rel_err(got, ref)[0] <= 1e-5
+
+
+def _tiny_case(nx, ny, nz, nsub, seed, mixed):
+    """Subnormal fields around 1e-310 (numerators far below 2^-960, where the corrected
+    reciprocal misrounds about one division in 400, some zero): the sweeps redo such tiles
+    with true divisions.  ``mixed``: only the z < nz/2 half is tiny, so tiles hold tiny and
+    ordinary lines together."""
+    rng = np.random.default_rng(seed)
+    init = rng.lognormal(0.0, 0.5, (nx, ny, nz, nsub)) * 1e-310
+    init[rng.uniform(size=init.shape) < 0.1] *= 1e-5
+    init[rng.uniform(size=init.shape) < 0.02] = 0.0
+    if mixed:
+        init[:, :, nz // 2:, :] = rng.lognormal(0.0, 0.5, (nx, ny, nz - nz // 2, nsub))
+    dvalue = [1e-312] + [0.0] * (nsub - 1)
+    return cf.make_case(nx, ny, nz, nsub, seed=seed, nagents=0, nsteps=3, init=init,
+                        dvalue=dvalue, dactive=[1] + [0] * (nsub - 1), with_agents=False)
+
+
+TINY = {
+    "tiny_40x36x30x2": (40, 36, 30, 2, 31, False),
+    "tiny_mixed_70x33x41x2": (70, 33, 41, 2, 32, True),
+    "tiny_long_x_700x5x6x1": (700, 5, 6, 1, 33, False),
+    "tiny_long_y_9x340x7x1": (9, 340, 7, 1, 34, True),
+    "tiny_long_z_5x6x700x1": (5, 6, 700, 1, 35, False),
+}
+
+
+@pytest.mark.parametrize("name", list(TINY))
+@pytest.mark.parametrize("launch", [None, (0, 8, 16)])
+def test_fp64_tiny_numerators_bitwise(name, launch):
+    """The exact-division redo (numerators below 2^-960, outside the corrected reciprocal's
+    range) in every sweep, with head recomputation on the long lines and small line stores,
+    bit for bit against the oracle (IEEE division, subnormals kept)."""
+    case = _tiny_case(*TINY[name])
+    print(f"[gpu] tiny numerators {name} launch={launch}", flush=True)
+    ref = orc.run(case)
+    assert np.count_nonzero(ref) and np.min(np.abs(ref[ref != 0])) < 2.0 ** -960
+    s = _solver(case, "fp64")
+    if launch is not None:
+        s.set_launch(-1, *launch)
+    s.run(case.nsteps)
+    got = s.download()
+    s.close()
+    if not np.array_equal(got, ref):
+        bad = np.argwhere(got != ref)
+        pytest.fail(f"{name}: {len(bad)} values differ, first {bad[:5].tolist()}")
