@@ -72,6 +72,7 @@ SIGNATURES = {
     "bfvmb_batch_set_launch": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_int, _c.c_int]),
     "bfvmb_batch_autotune": (_c.c_int, [_vp, _c.c_int, _ip]),
     "bfvmb_update_agent_positions": (_c.c_int, [_vp, _c.c_int, _vp, _dp, _c.c_int]),
+    "bfvmb_densities_at": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _c.c_int]),
     "bfvmb_batch_update_agent_positions": (_c.c_int, [_vp, _c.c_int, _c.c_int, _vp, _dp, _c.c_int]),
 }
 

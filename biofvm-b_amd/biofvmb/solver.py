@@ -132,6 +132,27 @@ class LODSolver:
         self._check(self._lib.bfvmb_set_agent_rates(self._h, n, L.iptr(v), L.dptr(vol), L.dptr(S), L.dptr(U), L.dptr(T),
                                                     None if a is None else L.bptr(a), float(dt)), "set_agent_rates")
 
+    def densities_at(self, voxel, out=None):
+        """Densities at the given voxels (reference numbering), [n][substrate] FP64, gathered
+        from the resident field (bfvmb_densities_at).  ``voxel`` may be a CUDA int32 tensor;
+        then ``out`` (a CUDA float64 tensor of n*nsub elements) is filled in stream order."""
+        if hasattr(voxel, "is_cuda") and voxel.is_cuda:
+            import torch
+            n = voxel.numel()
+            if out is None:
+                out = torch.empty((n, self.nsub), dtype=torch.float64, device=voxel.device)
+            if voxel.dtype != torch.int32 or not voxel.is_contiguous() or out.dtype != torch.float64 \
+                    or not out.is_contiguous() or out.numel() != n * self.nsub:
+                raise ValueError("device voxels: contiguous int32; out: contiguous float64 [n][nsub]")
+            self._check(self._lib.bfvmb_densities_at(self._h, n, ctypes.c_void_p(voxel.data_ptr()),
+                                                     ctypes.c_void_p(out.data_ptr()), 1), "densities_at")
+            return out
+        v = np.ascontiguousarray(voxel, dtype=np.int32).reshape(-1)
+        res = np.empty((v.size, self.nsub), dtype=np.float64)
+        self._check(self._lib.bfvmb_densities_at(self._h, v.size, ctypes.c_void_p(v.ctypes.data),
+                                                 ctypes.c_void_p(res.ctypes.data), 0), "densities_at")
+        return res
+
     def update_agent_positions(self, pos, bbox_min=(0.0, 0.0, 0.0)) -> None:
         """New positions ([n][3], host array or CUDA float64 tensor) of the agents last set:
         voxels and the grouping by voxel are recomputed on the device; agents outside the

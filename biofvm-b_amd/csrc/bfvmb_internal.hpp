@@ -152,6 +152,8 @@ int domain_set_bulk(Group *g, int i, const double *supply, const double *target,
 // regrouping on the device; free_remap releases that state
 int domain_update_agent_positions(Group *g, int i, int n, const double *pos, const double *bbox_min, int on_device);
 int domain_agent_order(Group *g, Domain &d);  // brings Domain::agent_order up to date
+// densities at listed voxels (field.hip); on_device: voxel and out are device pointers
+int domain_densities_at(Group *g, int i, int n, const int *voxel, double *out, int on_device);
 void free_remap(Domain &d);
 int group_bulk(Group *g, double dt);
 int group_steps(Group *g, int nsteps, bool with_agents, bool use_graph);

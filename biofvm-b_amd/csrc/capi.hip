@@ -139,6 +139,11 @@ int bfvmb_voxel_indices(int nx, int ny, int nz, double dx, const double *bbox_mi
     return BFVMB_OK;
 }
 
+int bfvmb_densities_at(bfvmb_ctx *ctx, int n, const int *voxel, double *out, int on_device)
+{
+    return ctx ? domain_densities_at(ctx, 0, n, voxel, out, on_device) : BFVMB_ERR_ARG;
+}
+
 int bfvmb_simulate_cell_sources_and_sinks(bfvmb_ctx *ctx)
 {
     if (!ctx)

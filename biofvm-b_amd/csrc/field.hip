@@ -118,6 +118,28 @@ __global__ __launch_bounds__(256) void k_soa_to_aos(const T *__restrict__ soa, d
     }
 }
 
+// Densities at listed voxels (reference numbering x*ny*nz + y*nz + z), out[k][substrate] in
+// FP64: the values Microenvironment::nearest_density_vector(voxel) points at
+// (BioFVM_microenvironment.cpp:989-992), read from the resident field.  A voxel outside the
+// grid reads as NaN (device-side lists are not checked on the host).
+template <class T>
+__global__ __launch_bounds__(256) void k_densities_at(const T *__restrict__ f, int nx, int ny, int nz, int nsub,
+                                                      const int *__restrict__ voxel, int n, double *__restrict__ out)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)n * nsub)
+        return;
+    const int k = (int)(i / nsub), s = (int)(i - (long long)k * nsub);
+    const long long v = voxel[k];
+    const long long nyz = (long long)ny * nz, nvox = nyz * nx;
+    if (v < 0 || v >= nvox) {
+        out[i] = __builtin_nan("");
+        return;
+    }
+    const long long x = v / nyz, rem = v - x * nyz, y = rem / nz, z = rem - y * nz;
+    out[i] = (double)f[(size_t)s * nvox + (size_t)((z * ny + y) * nx + x)];
+}
+
 template <class T>
 int launch_agents_t(Group *g)
 {
@@ -140,6 +162,48 @@ dim3 transpose_grid(const Domain &d)
 }
 
 }  // namespace
+
+int domain_densities_at(Group *g, int i, int n, const int *voxel, double *out, int on_device)
+{
+    if (i < 0 || (size_t)i >= g->doms.size() || n < 0 || (n > 0 && (!voxel || !out)))
+        return set_error(g, BFVMB_ERR_ARG, "densities_at: bad argument");
+    Domain &d = g->doms[i];
+    if (!on_device)
+        for (int k = 0; k < n; ++k)
+            if (voxel[k] < 0 || (size_t)voxel[k] >= d.nvox)
+                return set_error(g, BFVMB_ERR_ARG, "densities_at: voxel index out of range");
+    if (n == 0)
+        return BFVMB_OK;
+    const int *dv = voxel;
+    double *dout = out;
+    int *tv = nullptr;
+    double *tout = nullptr;
+    const size_t obytes = (size_t)n * d.nsub * sizeof(double);
+    int rc = BFVMB_OK;
+    if (!on_device &&
+        (!(rc = hip_check(g, hipMalloc((void **)&tv, n * sizeof(int)), "density voxels")) &&
+         !(rc = hip_check(g, hipMalloc((void **)&tout, obytes), "density output")) &&
+         !(rc = hip_check(g, hipMemcpyAsync(tv, voxel, n * sizeof(int), hipMemcpyHostToDevice, g->stream), "voxels")))) {
+        dv = tv;
+        dout = tout;
+    }
+    if (!rc) {
+        const unsigned grid = (unsigned)(((long long)n * d.nsub + 255) / 256);
+        if (g->precision == BFVMB_FP64)
+            hipLaunchKernelGGL(k_densities_at<double>, dim3(grid), dim3(256), 0, g->stream,
+                               static_cast<const double *>(d.field[g->cur]), d.nx, d.ny, d.nz, d.nsub, dv, n, dout);
+        else
+            hipLaunchKernelGGL(k_densities_at<float>, dim3(grid), dim3(256), 0, g->stream,
+                               static_cast<const float *>(d.field[g->cur]), d.nx, d.ny, d.nz, d.nsub, dv, n, dout);
+        rc = hip_check(g, hipGetLastError(), "densities_at launch");
+    }
+    if (!rc && !on_device &&
+        !(rc = hip_check(g, hipMemcpyAsync(out, tout, obytes, hipMemcpyDeviceToHost, g->stream), "densities")))
+        rc = hip_check(g, hipStreamSynchronize(g->stream), "densities sync");
+    (void)hipFree(tv);
+    (void)hipFree(tout);
+    return rc;
+}
 
 int launch_agents(Group *g)
 {

@@ -10,7 +10,10 @@
 // followed by simulate_diffusion_decay(dt).
 //
 // usage: bfvmb_driver <case.bin> <out.bin> [fp64|fp32] [--fused] [--time] [--mat out.mat]
-//                     [--bulk rates.bin]
+//                     [--bulk rates.bin] [--sample samples.bin]
+// --sample: at every snapshot, before the field is written, the densities at the agents'
+// voxels through Microenvironment::nearest_density_vectors (read from the device field),
+// [nagents][nsub] doubles per snapshot.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -58,11 +61,14 @@ int main(int argc, char **argv)
     bool fused = false, timing = false;
     const char *mat_path = nullptr;   // --mat PATH: write_to_matlab after the last step
     const char *bulk_path = nullptr;  // --bulk PATH: supply, target, uptake [nvox][nsub] doubles
+    const char *sample_path = nullptr;  // --sample PATH: densities at the agents' voxels per snapshot
     for (int i = 3; i < argc; ++i) {
         if (!std::strcmp(argv[i], "--mat") && i + 1 < argc)
             mat_path = argv[++i];
         else if (!std::strcmp(argv[i], "--bulk") && i + 1 < argc)
             bulk_path = argv[++i];
+        else if (!std::strcmp(argv[i], "--sample") && i + 1 < argc)
+            sample_path = argv[++i];
         else if (!std::strcmp(argv[i], "fp32"))
             precision = BFVMB_FP32;
         else if (!std::strcmp(argv[i], "--fused"))
@@ -199,6 +205,16 @@ int main(int argc, char **argv)
             ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ta).count();
             done = s;
             while (next < nsnap && snaps[next] == s) {
+                if (sample_path) {
+                    FILE *sf = std::fopen(sample_path, next == 0 ? "wb" : "ab");
+                    if (!sf) {
+                        std::perror(sample_path);
+                        return 3;
+                    }
+                    const std::vector<double> d = M.nearest_density_vectors(voxel);
+                    std::fwrite(d.data(), sizeof(double), d.size(), sf);
+                    std::fclose(sf);
+                }
                 std::fwrite(&s, sizeof(int), 1, out);
                 std::fwrite(M.density_data(), sizeof(double), nvox * nsub, out);
                 ++next;

@@ -296,6 +296,24 @@ double *Microenvironment::density_data()
 
 double *Microenvironment::density_vector(int n) { return density_data() + (size_t)n * number_of_densities(); }
 
+std::vector<double> Microenvironment::nearest_density_vectors(const std::vector<int> &voxels)
+{
+    const size_t ns = (size_t)number_of_densities();
+    std::vector<double> out(voxels.size() * ns);
+    if (!host_stale_) {  // the host copy is current (or newer): read it
+        for (size_t k = 0; k < voxels.size(); ++k) {
+            const int v = voxels[k];
+            if (v < 0 || v >= number_of_voxels())
+                throw Error(BFVMB_ERR_ARG, "nearest_density_vectors: voxel index out of range");
+            for (size_t s = 0; s < ns; ++s)
+                out[k * ns + s] = host_[(size_t)v * ns + s];
+        }
+        return out;
+    }
+    check(bfvmb_densities_at(ctx_, (int)voxels.size(), voxels.data(), out.data(), 0), "bfvmb_densities_at");
+    return out;
+}
+
 void Microenvironment::ensure_solver(double dt)
 {
     ensure_context();

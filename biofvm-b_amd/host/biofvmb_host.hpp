@@ -111,6 +111,10 @@ public:
     double *operator()(int n) { return density_vector(n); }
     double *operator()(int i, int j, int k) { return density_vector(i, j, k); }
     double *density_data();  // the whole host field
+    // densities ([k][substrate]) of the given voxels, read-only: what nearest_density_vector
+    // (BioFVM_microenvironment.cpp:984-992) points at, gathered from the device field when it
+    // is the newer copy, so that sampling a few voxels does not download the whole field
+    std::vector<double> nearest_density_vectors(const std::vector<int> &voxels);
     void sync_to_host();
     void sync_to_device();
 

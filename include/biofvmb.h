@@ -88,6 +88,15 @@ int bfvmb_set_agent_rates(bfvmb_ctx *ctx, int n, const int *voxel, const double 
 int bfvmb_voxel_indices(int nx, int ny, int nz, double dx, const double *bbox_min, int n,
                         const double *pos, int *voxel);
 
+/* Densities at n voxels (reference numbering), out[k][substrate] in FP64: what
+ * Microenvironment::nearest_density_vector(voxel) / density_vector(n) point at
+ * (BioFVM_microenvironment.cpp:984-1015) and Basic_Agent::nearest_density_vector reads
+ * (BioFVM_basic_agent.cpp:250-253), gathered from the resident field without downloading
+ * it.  on_device != 0: voxel and out are device pointers (stream-ordered, no wait; a voxel
+ * outside the grid reads as NaN); otherwise host arrays, indices checked, blocks until out is
+ * filled. */
+int bfvmb_densities_at(bfvmb_ctx *ctx, int n, const int *voxel, double *out, int on_device);
+
 /* Moving agents: new positions ([n][3], micrometres; grid lower corner bbox_min, NULL = the
  * origin) of the n agents last given to bfvmb_set_agents / bfvmb_set_agent_rates, as
  * Basic_Agent::update_voxel_index computes their voxels (BioFVM_basic_agent.cpp:137-153 with
