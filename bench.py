@@ -243,12 +243,13 @@ def bench_agent_moves(torch, stream, n=400, nagents=1_000_000, reps=10, seed=11)
     return out
 
 
-def cpu_baseline():
+def cpu_baseline(repeats=3):
     """The reference (oracle/_ref) on a bounded sample: 200^3 x 4 with 125k agents (1/8 of
-    the domain, same densities), 8 steps; falls back to the C restatement (1 thread)."""
+    the domain, same densities), 16 steps, median of three runs (≈20 s of CPU work on 16
+    threads); falls back to the C restatement (1 thread)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import casefile as cf
-    n, steps = 200, 8
+    n, steps = 200, 16
     init, pos, vol, S, U, T = make_inputs(n=n, nagents=N_AGENTS // 8, seed=7)
     case = cf.Case(n, n, n, NSUB, DX, DT, np.array(D), np.array(DECAY), np.array([38.0, 0, 0, 0]),
                    np.array([1, 0, 0, 0]), init, pos, vol, S, U, T, steps, [], True)
@@ -257,15 +258,18 @@ def cpu_baseline():
     except AttributeError:
         threads = os.cpu_count() or 1
     threads = max(1, min(threads, 16))
-    sample = f"{n}^3x{NSUB}, {N_AGENTS // 8} agents, {steps} steps (sources/sinks + LOD)"
+    sample = f"{n}^3x{NSUB}, {N_AGENTS // 8} agents, {steps} steps (sources/sinks + LOD), median of {repeats} runs"
     if os.path.exists(cf.REF_DRIVER):
         try:
-            with tempfile.TemporaryDirectory() as d:
-                out = cf.run_reference(case, d, solver="lod", threads=threads, timing=True)
-            rec = json.loads(out["stdout"].strip().splitlines()[-1])
-            vs = n ** 3 * NSUB * steps / (rec["ms_total"] / 1e3)
-            return {"value": vs, "unit": "voxel-substrate updates/s", "cores": threads, "kind": "reference",
-                    "sample": sample + f", 1 rank x {threads} OpenMP threads, LOD_3D solver"}
+            rates = []
+            for _ in range(repeats):
+                with tempfile.TemporaryDirectory() as d:
+                    out = cf.run_reference(case, d, solver="lod", threads=threads, timing=True)
+                rec = json.loads(out["stdout"].strip().splitlines()[-1])
+                rates.append(n ** 3 * NSUB * steps / (rec["ms_total"] / 1e3))
+                log(f"cpu baseline run: {rates[-1]:.3e} vs/s")
+            return {"value": float(np.median(rates)), "unit": "voxel-substrate updates/s", "cores": threads,
+                    "kind": "reference", "sample": sample + f", 1 rank x {threads} OpenMP threads, LOD_3D solver"}
         except Exception as e:  # reference binary not runnable here: use the restatement
             log(f"reference CPU run failed ({e}); timing the C restatement instead")
     import oracle as orc
