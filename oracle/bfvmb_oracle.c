@@ -2,7 +2,7 @@
  * bfvmb_oracle.c -- CPU restatement of BioFVM-B's FP64 constant-coefficient LOD
  * diffusion-decay step and its agent source/sink update.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the parity checker for the HIP library in
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity oracle for the HIP library in
  * biofvm-b_amd/.  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg
  * load it; the product path never links, calls or falls back to it.
  *

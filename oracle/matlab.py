@@ -1,5 +1,5 @@
 """MAT v4 "multiscale_microenvironment" files: numpy restatement of the reference's writer
-and reader.  Test infrastructure (the checker of the device writer/reader), not product.
+and reader.  Test infrastructure (the oracle of the device writer/reader), not product.
 
 Writer: Microenvironment::write_to_matlab (BioFVM_microenvironment.cpp:1079-1104) through
 write_matlab4_header (BioFVM_matlab.cpp:535-579); the MPI-IO variant

@@ -1,7 +1,7 @@
 """ctypes front end of the C restatement (``oracle/bfvmb_oracle.c``) -- test infrastructure.
 
 Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg use this module,
-and only as the checker; the product path (biofvm-b_amd/) never imports it.
+and only to compare against; the product path (biofvm-b_amd/) never imports it.
 """
 from __future__ import annotations
 

@@ -157,7 +157,7 @@ int main(int argc, char **argv)
         M.diffusion_decay_solver = bfvmb_hip_LOD_3D;
 #else
     if (hip) {
-        fprintf(stderr, "this build has no GPU plugin\n");
+        fprintf(stderr, "built without the GPU plugin\n");
         return 2;
     }
 #endif

@@ -26,9 +26,9 @@ import casefile as cf  # noqa: E402
 def cases():
     c1 = cf.make_case(50, 50, 50, 1, seed=10, nagents=1000, nsteps=100, snaps=[1, 100], D=[1e5], decay=[0.1],
                       dvalue=[38.0], dactive=[1], init=38.0)
-    c1.U[:] = 10.0  # config 1: uptake 10/min, secretion 0
+    c1.U[:] = 10.0  # the plain 50^3 x 1 case: uptake 10/min, secretion 0
     c1.S[:] = 0.0
-    yield "config1_50cube", c1
+    yield "plain_50cube", c1
     yield "aniso_17x23x31x3", cf.make_case(17, 23, 31, 3, seed=1, nagents=200, nsteps=5, snaps=[1, 5],
                                            shared_voxels=True, outside=3)
     yield "nx1_1x5x7x2", cf.make_case(1, 5, 7, 2, seed=2, nagents=10, nsteps=3)
@@ -152,7 +152,7 @@ def load(name):
     return case, ref
 
 
-NAMES = ["config1_50cube", "aniso_17x23x31x3", "nx1_1x5x7x2", "ny1_2x1x9x4", "nz1_6x4x1x2", "one_voxel_1x1x1x3",
+NAMES = ["plain_50cube", "aniso_17x23x31x3", "nx1_1x5x7x2", "ny1_2x1x9x4", "nz1_6x4x1x2", "one_voxel_1x1x1x3",
          "conserve_20x18x16x2", "multi_24x20x28x5", "defaults_12x10x14x2", "long_x_300x17x19x1"]
 
 MAT_NAMES = ["mat_offset_13x9x11x3", "mat_1x1x2x1"]

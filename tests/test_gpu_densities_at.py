@@ -65,7 +65,7 @@ def test_device_lists_and_out_of_range():
     s.close()
 
 
-@pytest.mark.parametrize("name", ["aniso_17x23x31x3", "multi_24x20x28x5", "config1_50cube"])
+@pytest.mark.parametrize("name", ["aniso_17x23x31x3", "multi_24x20x28x5", "plain_50cube"])
 def test_host_layer_samples_match_snapshots(name):
     """Microenvironment::nearest_density_vectors of the C++ host layer (read from the device
     after each run segment) against the field the same driver writes at that snapshot."""

@@ -2,7 +2,7 @@
 
 The FP64 path must be bit-identical to the C restatement (itself bit-identical to the
 compiled reference, see test_oracle.py); the FP32 path is held to 1e-5 max relative
-error against the FP64 oracle, with the error printed.
+error against the FP64 oracle (the tolerance of the FP32 storage path), with the error printed.
 """
 import numpy as np
 import pytest
@@ -52,7 +52,7 @@ CASES = {
     "long_z_5x6x700x2": dict(nx=5, ny=6, nz=700, nsub=2, seed=17, nagents=50, nsteps=3),
     "all_dirichlet_64x48x40x4": dict(nx=64, ny=48, nz=40, nsub=4, seed=8, nagents=2000, nsteps=8, dactive=[1, 1, 1, 1]),
     "no_dirichlet_40x41x42x2": dict(nx=40, ny=41, nz=42, nsub=2, seed=9, nagents=500, nsteps=8, dactive=[0, 0]),
-    "config1_50cube_100steps": dict(nx=50, ny=50, nz=50, nsub=1, seed=10, nagents=1000, nsteps=100, D=[1e5], decay=[0.1],
+    "plain_50cube_100steps": dict(nx=50, ny=50, nz=50, nsub=1, seed=10, nagents=1000, nsteps=100, D=[1e5], decay=[0.1],
                                     dvalue=[38.0], dactive=[1], init=38.0),
 }
 
@@ -60,7 +60,7 @@ CASES = {
 def _case(name):
     kw = dict(CASES[name])
     c = cf.make_case(**kw)
-    if name == "config1_50cube_100steps":  # uptake 10/min, secretion 0
+    if name == "plain_50cube_100steps":  # uptake 10/min, secretion 0
         c.U[:] = 10.0
         c.S[:] = 0.0
     return c

@@ -1,4 +1,4 @@
-"""Pins the C restatement (oracle/bfvmb_oracle.c) before it is trusted as the checker.
+"""Pins the C restatement (oracle/bfvmb_oracle.c) before it is trusted as the oracle.
 
 1. Bit for bit against the reference's own outputs (tests/golden/, written by the compiled
    reference: oracle/_ref/ref_driver via tests/golden/make_golden.py).
@@ -84,7 +84,7 @@ def test_lod_step_equals_dense_solves(shape):
 
 
 def test_dense_check_catches_a_wrong_sign():
-    """A planted mistake in the checker's inputs (decay sign flipped) must fail the dense check."""
+    """A planted mistake in the oracle's inputs (decay sign flipped) must fail the dense check."""
     case = cf.make_case(6, 5, 4, 2, seed=3, nsteps=1, with_agents=False, decay=[0.5, 0.2])
     value, active = cf.effective_dirichlet(case)
     want = _dense_lod_step(case.init, case.D, -case.decay, case.dt, case.dx, value, active)
