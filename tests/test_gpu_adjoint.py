@@ -262,3 +262,99 @@ def test_torch_autograd_field_and_rates():
         worst = max(worst, rel if abs(fd) > 1e-12 else 0.0)
         assert abs(fd - ad) <= 1e-6 * abs(fd) or fd == ad == 0.0, (kind, idx, fd, ad)
     print(f"32 coordinates: worst relative error {worst:.2e}")
+
+
+def _bulk_case():
+    c = _case()
+    rng = np.random.default_rng(21)
+    shape = c.init.shape
+    c.bulk = (rng.uniform(0.0, 40.0, shape), rng.uniform(0.0, 3.0, shape), rng.uniform(0.0, 60.0, shape))
+    return c
+
+
+def test_gradients_with_bulk_rates_match_finite_differences():
+    """With bulk sources/sinks in every step (rho <- (rho + dt T S)/(1 + dt (U + S)) before the
+    agents), the reverse pass applies its adjoint and records the agents' inputs after it."""
+    c = _bulk_case()
+    tgt = _target(c)
+    voxel, t1, t2 = orc.agent_tables(c)
+    print("[gpu] adjoint with bulk rates", flush=True)
+    s = _solver(c)
+    s.set_bulk_rates(*c.bulk)
+    L, g0, g1, g2, rT = _adjoint(s, c.nsteps, tgt, every=3)
+    s.close()
+    assert np.array_equal(rT, orc.run(c))
+    rng = np.random.default_rng(9)
+    f0 = lambda r: orc.run(c, rho=r)  # noqa: E731
+    f1 = lambda x: orc.run(c, tables=(x, t2))  # noqa: E731
+    f2 = lambda x: orc.run(c, tables=(t1, x))  # noqa: E731
+    for k in range(3):
+        v = rng.standard_normal(c.init.shape)
+        fd, ad = _fd_dir(f0, c.init, v, 1e-3, tgt), float(np.sum(g0 * v))
+        print(f"bulk rho0 dir {k}: fd {fd:.12e} adjoint {ad:.12e}")
+        assert abs(fd - ad) <= 1e-6 * abs(fd)
+        v = rng.standard_normal(t1.shape) * np.abs(t1).max()
+        fd, ad = _fd_dir(f1, t1, v, 0.5, tgt), float(np.sum(g1 * v))
+        assert abs(fd - ad) <= 1e-6 * abs(fd)
+        v = rng.standard_normal(t2.shape) * np.abs(t2).max()
+        fd, ad = _fd_dir(f2, t2, v, 1e-5, tgt), float(np.sum(g2 * v))
+        assert abs(fd - ad) <= 1e-6 * abs(fd)
+
+
+def test_repeated_backward_calls_replay_the_same_graphs():
+    """The reverse-pass state and its segment graphs are kept with the record: several seeds
+    in a row, with eager work on the context in between, give what fresh records give."""
+    c = _case(nsteps=11)
+    tgt = _target(c)
+    print("[gpu] adjoint repeated backward", flush=True)
+    s = _solver(c)
+    s.adjoint_forward(c.nsteps, 4)
+    rT = s.download()
+    seeds = [2.0 * (rT - tgt), np.random.default_rng(4).standard_normal(rT.shape), 2.0 * (rT - tgt)]
+    outs = [s.adjoint_backward(w) for w in seeds]
+    s.close()
+    for k, w in enumerate(seeds):
+        f = _solver(c)
+        f.adjoint_forward(c.nsteps, 4)
+        want = f.adjoint_backward(w)
+        f.close()
+        for a, b in zip(outs[k], want):
+            assert np.array_equal(a, b)
+
+
+def test_new_records_reuse_state_only_when_valid():
+    """Records taken one after another on one context (other start parity, other checkpoint
+    interval, new agent tables, bulk rates added) give what a fresh context gives."""
+    c = _case(nsteps=9)
+    tgt = _target(c)
+    voxel, t1, t2 = orc.agent_tables(c)
+    print("[gpu] adjoint records in a row", flush=True)
+
+    def fresh(nsteps, every, tables=None, bulk=None, steps_before=0):
+        f = _solver(c, tables=tables)
+        if bulk is not None:
+            f.set_bulk_rates(*bulk)
+        f.run(steps_before)
+        out = _adjoint(f, nsteps, tgt, every=every)
+        f.close()
+        return out
+
+    bulk = _bulk_case().bulk
+    s = _solver(c)
+    plan = [(9, 3, None, None, 0), (9, 3, None, None, 1), (5, 2, None, None, 0), (9, 3, (t1 * 1.5, t2), None, 0),
+            (9, 3, (t1 * 1.5, t2), bulk, 0)]
+    done = 0
+    for nsteps, every, tables, bk, before in plan:
+        s.upload(c.init)
+        if tables is not None:
+            s.set_agents(voxel, *tables)
+        if bk is not None:
+            s.set_bulk_rates(*bk)
+        s.run(before)
+        got = _adjoint(s, nsteps, tgt, every=every)
+        want = fresh(nsteps, every, tables, bk, before)
+        for a, b in zip(got, want):
+            assert np.array_equal(a, b), (nsteps, every, before)
+        done += 1
+    s.close()
+    assert done == len(plan)
