@@ -36,8 +36,11 @@ def lod_run(solver, rho0, voxel, temp1, temp2, nsteps: int, checkpoint_every: in
     """nsteps of (agent sources/sinks, LOD step) from ``rho0`` [nx,ny,nz,nsub] (FP64) on the
     context of ``solver`` (an LODSolver), with agents at ``voxel`` and tables temp1/temp2
     [n,nsub].  The context's field is rho_T afterwards; backward needs the context unchanged
-    in between (another forward call or step makes it raise)."""
+    in between (another forward call or step makes it raise).  A CUDA ``rho0`` keeps the
+    field and its gradient in HBM (the context then runs on torch's current stream)."""
     load()
+    if rho0.is_cuda:  # the device path is stream-ordered on the context's stream
+        solver.set_stream(torch.cuda.current_stream().cuda_stream)
     return torch.ops.biofvmb.lod_run(rho0, voxel, temp1, temp2, solver.handle, int(nsteps), int(checkpoint_every))
 
 

@@ -102,15 +102,34 @@ class LODSolver:
         a = np.ascontiguousarray(np.broadcast_to(active, (self.nsub,)), dtype=np.uint8)
         self._check(self._lib.bfvmb_set_dirichlet(self._h, L.dptr(v), L.bptr(a)), "set_dirichlet")
 
+    def _device_f64(self, t, what):
+        import torch
+        if t.dtype != torch.float64 or not t.is_contiguous() or t.numel() != int(np.prod(self.shape)):
+            raise ValueError(f"{what}: a contiguous float64 CUDA tensor of {tuple(self.shape)} elements")
+        torch.cuda.current_stream().synchronize()  # the tensor's producers are done
+        return ctypes.c_void_p(t.data_ptr())
+
     def upload(self, rho) -> None:
+        """Field [x][y][z][substrate]: a host array, or a CUDA float64 tensor (no host copy)."""
+        if hasattr(rho, "is_cuda") and rho.is_cuda:
+            self._check(self._lib.bfvmb_upload_density_device(self._h, self._device_f64(rho, "upload")),
+                        "upload_density_device")
+            self.sync()
+            return
         r = np.ascontiguousarray(rho, dtype=np.float64)
         assert r.size == np.prod(self.shape)
         self._check(self._lib.bfvmb_upload_density(self._h, L.dptr(r)), "upload_density")
 
-    def download(self) -> np.ndarray:
-        out = np.empty(self.shape, dtype=np.float64)
-        self._check(self._lib.bfvmb_download_density(self._h, L.dptr(out)), "download_density")
-        return out
+    def download(self, out=None):
+        """The field as a host array, or into ``out`` (a CUDA float64 tensor, no host copy)."""
+        if out is not None and hasattr(out, "is_cuda") and out.is_cuda:
+            self._check(self._lib.bfvmb_download_density_device(self._h, self._device_f64(out, "download")),
+                        "download_density_device")
+            self.sync()
+            return out
+        res = np.empty(self.shape, dtype=np.float64)
+        self._check(self._lib.bfvmb_download_density(self._h, L.dptr(res)), "download_density")
+        return res
 
     def set_agents(self, voxel, temp1, temp2, active=None) -> None:
         v = np.ascontiguousarray(voxel, dtype=np.int32)
@@ -228,7 +247,21 @@ class LODSolver:
 
     def adjoint_backward(self, seed):
         """Gradients (d/drho_0 [x][y][z][substrate], d/dtemp1, d/dtemp2 [agent][substrate]) of
-        sum(seed * rho_T) for the run recorded by ``adjoint_forward``."""
+        sum(seed * rho_T) for the run recorded by ``adjoint_forward``.  A CUDA float64 seed
+        gives CUDA gradients, with no host transfer (bfvmb_adjoint_backward_device)."""
+        if hasattr(seed, "is_cuda") and seed.is_cuda:
+            import torch
+            na = getattr(self, "_nagents", 0)
+            p = self._device_f64(seed, "adjoint_backward")
+            g0 = torch.empty(tuple(self.shape), dtype=torch.float64, device=seed.device)
+            g1 = torch.empty((na, self.nsub), dtype=torch.float64, device=seed.device)
+            g2 = torch.empty((na, self.nsub), dtype=torch.float64, device=seed.device)
+            torch.cuda.current_stream().synchronize()
+            ptr = (lambda t: ctypes.c_void_p(t.data_ptr()) if t.numel() else None)
+            self._check(self._lib.bfvmb_adjoint_backward_device(self._h, p, ptr(g0), ptr(g1), ptr(g2)),
+                        "adjoint_backward_device")
+            self.sync()
+            return g0, g1, g2
         w = np.ascontiguousarray(seed, dtype=np.float64)
         assert w.size == np.prod(self.shape)
         na = getattr(self, "_nagents", 0)

@@ -101,6 +101,19 @@ __global__ __launch_bounds__(256) void k_bulk_adjoint(T *lam, const double *us, 
     lam[e] = (T)((double)lam[e] / (us[e] * dt + 1.0));
 }
 
+// rate gradients from the sorted [nsub][nact] accumulators to the caller's [agent][nsub]
+// order (order[k] = caller index of sorted agent k); inactive agents keep the zeros the
+// output was cleared to
+__global__ __launch_bounds__(256) void k_scatter_rates(const double *__restrict__ g, const int *__restrict__ order,
+                                                       int nact, int nsub, double *__restrict__ out)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)nact * nsub)
+        return;
+    const int s = (int)(i / nact), k = (int)(i - (long long)s * nact);
+    out[(size_t)order[k] * nsub + s] = g[i];
+}
+
 }  // namespace
 
 // Device state of the reverse pass, created by the first adjoint_backward of a tape and
@@ -124,6 +137,7 @@ struct Reverse {
     void *zeros = nullptr;  // Dirichlet value 0 of the adjoint sweeps
     double *r0 = nullptr;   // agent inputs per step of a segment [every][nsub][nseg]
     double *rbuf = nullptr, *g1 = nullptr, *g2 = nullptr;  // [nsub][nact]
+    int *order = nullptr;   // device copy of the agent order, for device-side gradient outputs
     Group adj;
     std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;  // (segment steps, adjoint parity,
                                                                   // primal parity)
@@ -149,7 +163,7 @@ void free_reverse(Reverse *r)
     for (auto &kv : r->graphs)
         (void)hipGraphExecDestroy(kv.second);
     for (void *p : {r->lam[0], r->lam[1], r->stage, r->zeros, (void *)r->r0, (void *)r->rbuf, (void *)r->g1,
-                    (void *)r->g2, r->adj.d_desc, (void *)r->adj.d_prefix})
+                    (void *)r->g2, (void *)r->order, r->adj.d_desc, (void *)r->adj.d_prefix})
         (void)hipFree(p);
     delete r;
 }
@@ -241,8 +255,10 @@ int make_reverse(Group *g, Tape &tp)
     return BFVMB_OK;
 }
 
+// on_device: seed and the three outputs are device pointers; the outputs are written in
+// stream order (no wait at the end)
 template <class T>
-int backward_t(Group *g, const double *seed, double *grad_rho0, double *grad_t1, double *grad_t2)
+int backward_t(Group *g, const double *seed, double *grad_rho0, double *grad_t1, double *grad_t2, bool on_device)
 {
     Tape &tp = *g->tape;
     Domain &d = g->doms[0];
@@ -257,11 +273,12 @@ int backward_t(Group *g, const double *seed, double *grad_rho0, double *grad_t1,
     Group &adj = rv.adj;
     Domain &ad = adj.doms[0];
     const size_t abytes = (size_t)nact * ns * sizeof(double);
-    if ((rc = hip_check(g, hipMemcpyAsync(rv.stage, seed, nel * sizeof(double), hipMemcpyHostToDevice, g->stream),
+    if (!on_device &&
+        (rc = hip_check(g, hipMemcpyAsync(rv.stage, seed, nel * sizeof(double), hipMemcpyHostToDevice, g->stream),
                         "seed upload")))
         return rc;
     adj.cur = 0;
-    if ((rc = launch_aos_to_soa(g, d, static_cast<const double *>(rv.stage), rv.lam[0])))
+    if ((rc = launch_aos_to_soa(g, d, on_device ? seed : static_cast<const double *>(rv.stage), rv.lam[0])))
         return rc;
     if (nseg > 0 && ((rc = hip_check(g, hipMemsetAsync(rv.g1, 0, abytes, g->stream), "gradient init")) ||
                      (rc = hip_check(g, hipMemsetAsync(rv.g2, 0, abytes, g->stream), "gradient init"))))
@@ -350,6 +367,42 @@ int backward_t(Group *g, const double *seed, double *grad_rho0, double *grad_t1,
         adj.cur = a0 ^ (L & 1);
     }
     // outputs
+    if (on_device) {
+        const size_t obytes = (size_t)d.agents_given * ns * sizeof(double);
+        if (grad_rho0 && (rc = launch_soa_to_aos(g, d, ad.field[adj.cur], grad_rho0)))
+            return rc;
+        if (obytes && (grad_t1 || grad_t2)) {
+            if (nseg > 0 && !rv.order) {  // the order is fixed for the record's agent tables
+                if ((rc = domain_agent_order(g, d)) ||
+                    (rc = rev_alloc(g, (void **)&rv.order, (size_t)nact * sizeof(int))) ||
+                    (rc = hip_check(g, hipMemcpyAsync(rv.order, d.agent_order.data(), (size_t)nact * sizeof(int),
+                                                      hipMemcpyHostToDevice, g->stream), "agent order")) ||
+                    (rc = hip_check(g, hipStreamSynchronize(g->stream), "agent order")))
+                    return rc;
+            }
+            const unsigned sgrid = (unsigned)(((long long)nact * ns + 255) / 256);
+            double *outs[2] = {grad_t1, grad_t2};
+            const double *accs[2] = {rv.g1, rv.g2};
+            for (int k = 0; k < 2; ++k) {
+                if (!outs[k])
+                    continue;
+                if ((rc = hip_check(g, hipMemsetAsync(outs[k], 0, obytes, g->stream), "gradient clear")))
+                    return rc;
+                if (nseg > 0 && nact > 0)
+                    hipLaunchKernelGGL(k_scatter_rates, dim3(sgrid), dim3(256), 0, g->stream, accs[k], rv.order, nact,
+                                       ns, outs[k]);
+            }
+            if ((rc = hip_check(g, hipGetLastError(), "gradient scatter")))
+                return rc;
+        }
+        const int end = tp.start ^ (nsteps & 1);
+        if ((rc = hip_check(g, hipMemcpyAsync(field(end), tp.fin, fbytes, hipMemcpyDeviceToDevice, g->stream),
+                            "restore")))
+            return rc;
+        g->cur = end;
+        g->epoch = tp.epoch;
+        return BFVMB_OK;
+    }
     if (grad_rho0 &&
         ((rc = launch_soa_to_aos(g, d, ad.field[adj.cur], static_cast<double *>(rv.stage))) ||
          (rc = hip_check(g, hipMemcpyAsync(grad_rho0, rv.stage, nel * sizeof(double), hipMemcpyDeviceToHost,
@@ -461,7 +514,8 @@ int adjoint_forward(Group *g, int nsteps, int every)
     return BFVMB_OK;
 }
 
-int adjoint_backward(Group *g, const double *seed, double *grad_rho0, double *grad_t1, double *grad_t2)
+int adjoint_backward(Group *g, const double *seed, double *grad_rho0, double *grad_t1, double *grad_t2,
+                     bool on_device)
 {
     if (!seed)
         return set_error(g, BFVMB_ERR_ARG, "adjoint_backward: seed is NULL");
@@ -469,8 +523,8 @@ int adjoint_backward(Group *g, const double *seed, double *grad_rho0, double *gr
         return set_error(g, BFVMB_ERR_STATE,
                          "adjoint_backward: no adjoint_forward record for the current field, coefficients and agents");
     if (g->precision == BFVMB_FP64)
-        return backward_t<double>(g, seed, grad_rho0, grad_t1, grad_t2);
-    return backward_t<float>(g, seed, grad_rho0, grad_t1, grad_t2);
+        return backward_t<double>(g, seed, grad_rho0, grad_t1, grad_t2, on_device);
+    return backward_t<float>(g, seed, grad_rho0, grad_t1, grad_t2, on_device);
 }
 
 }  // namespace bfvmb
@@ -488,4 +542,12 @@ extern "C" int bfvmb_adjoint_backward(bfvmb_ctx *ctx, const double *seed, double
     if (!ctx)
         return BFVMB_ERR_ARG;
     return bfvmb::adjoint_backward(ctx, seed, grad_rho0, grad_temp1, grad_temp2);
+}
+
+extern "C" int bfvmb_adjoint_backward_device(bfvmb_ctx *ctx, const double *seed, double *grad_rho0,
+                                             double *grad_temp1, double *grad_temp2)
+{
+    if (!ctx)
+        return BFVMB_ERR_ARG;
+    return bfvmb::adjoint_backward(ctx, seed, grad_rho0, grad_temp1, grad_temp2, true);
 }

@@ -154,6 +154,8 @@ int domain_update_agent_positions(Group *g, int i, int n, const double *pos, con
 int domain_agent_order(Group *g, Domain &d);  // brings Domain::agent_order up to date
 // densities at listed voxels (field.hip); on_device: voxel and out are device pointers
 int domain_densities_at(Group *g, int i, int n, const int *voxel, double *out, int on_device);
+int domain_upload_device(Group *g, int i, const double *d_rho);   // group.hip, stream-ordered
+int domain_download_device(Group *g, int i, double *d_rho);
 void free_remap(Domain &d);
 int group_bulk(Group *g, double dt);
 int group_steps(Group *g, int nsteps, bool with_agents, bool use_graph);
@@ -163,7 +165,8 @@ int group_set_launch(Group *g, int sweep, long long grid, int bpc, int ring);
 int group_autotune(Group *g, int reps, int *chosen);
 // Adjoint (adjoint.hip): forward run recording checkpoints, then the reverse pass.
 int adjoint_forward(Group *g, int nsteps, int every);
-int adjoint_backward(Group *g, const double *seed, double *grad_rho0, double *grad_t1, double *grad_t2);
+int adjoint_backward(Group *g, const double *seed, double *grad_rho0, double *grad_t1, double *grad_t2,
+                     bool on_device = false);
 void adjoint_free(Group *g);
 // Density gradients of a single-domain group (gradient.hip).
 int domain_compute_gradients(Group *g);

@@ -70,6 +70,16 @@ int bfvmb_upload_density(bfvmb_ctx *ctx, const double *rho) { return ctx ? domai
 
 int bfvmb_download_density(bfvmb_ctx *ctx, double *rho) { return ctx ? domain_download(ctx, 0, rho) : BFVMB_ERR_ARG; }
 
+int bfvmb_upload_density_device(bfvmb_ctx *ctx, const double *rho)
+{
+    return ctx ? domain_upload_device(ctx, 0, rho) : BFVMB_ERR_ARG;
+}
+
+int bfvmb_download_density_device(bfvmb_ctx *ctx, double *rho)
+{
+    return ctx ? domain_download_device(ctx, 0, rho) : BFVMB_ERR_ARG;
+}
+
 int bfvmb_set_agents(bfvmb_ctx *ctx, int n, const int *voxel, const double *temp1, const double *temp2,
                      const uint8_t *active)
 {

@@ -346,6 +346,24 @@ int domain_upload(Group *g, int i, const double *rho)
     return hip_check(g, hipStreamSynchronize(g->stream), "upload sync");
 }
 
+// the same transposes from / into a caller's device buffer, in stream order (no wait)
+int domain_upload_device(Group *g, int i, const double *d_rho)
+{
+    if (bad_index(g, i) || !d_rho)
+        return set_error(g, BFVMB_ERR_ARG, "upload_density_device: bad argument");
+    Domain &d = g->doms[i];
+    g->epoch++;
+    return launch_aos_to_soa(g, d, d_rho, d.field[g->cur]);
+}
+
+int domain_download_device(Group *g, int i, double *d_rho)
+{
+    if (bad_index(g, i) || !d_rho)
+        return set_error(g, BFVMB_ERR_ARG, "download_density_device: bad argument");
+    Domain &d = g->doms[i];
+    return launch_soa_to_aos(g, d, d.field[g->cur], d_rho);
+}
+
 int domain_download(Group *g, int i, double *rho)
 {
     if (bad_index(g, i) || !rho)

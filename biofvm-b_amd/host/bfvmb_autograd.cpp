@@ -58,11 +58,17 @@ struct LodRun : public torch::autograd::Function<LodRun> {
         const Tensor t1 = host_f64(temp1).reshape({n, nsub}), t2 = host_f64(temp2).reshape({n, nsub});
         check(h, bfvmb_set_agents(h, (int)n, v.data_ptr<int>(), t1.data_ptr<double>(), t2.data_ptr<double>(), nullptr),
               "set_agents");
-        const Tensor r0 = host_f64(rho0);
-        check(h, bfvmb_upload_density(h, r0.data_ptr<double>()), "upload_density");
+        // a CUDA field stays on the device (stream-ordered on the context's stream, which the
+        // Python wrapper sets to torch's current stream); otherwise through host memory
+        const bool dev = rho0.is_cuda();
+        const Tensor r0 = dev ? rho0.detach().to(torch::kFloat64).contiguous() : host_f64(rho0);
+        check(h, dev ? bfvmb_upload_density_device(h, r0.data_ptr<double>()) : bfvmb_upload_density(h, r0.data_ptr<double>()),
+              "upload_density");
         check(h, bfvmb_adjoint_forward(h, (int)nsteps, (int)every), "adjoint_forward");
         Tensor out = torch::empty_like(r0);
-        check(h, bfvmb_download_density(h, out.data_ptr<double>()), "download_density");
+        check(h, dev ? bfvmb_download_density_device(h, out.data_ptr<double>())
+                     : bfvmb_download_density(h, out.data_ptr<double>()),
+              "download_density");
         ctx->saved_data["handle"] = handle;
         ctx->saved_data["nagents"] = n;
         ctx->saved_data["nsub"] = nsub;
@@ -76,12 +82,15 @@ struct LodRun : public torch::autograd::Function<LodRun> {
     {
         bfvmb_ctx *h = as_ctx(ctx->saved_data["handle"].toInt());
         const int64_t n = ctx->saved_data["nagents"].toInt(), nsub = ctx->saved_data["nsub"].toInt();
-        const Tensor w = host_f64(grads[0]);
+        const bool dev = grads[0].is_cuda();
+        const Tensor w = dev ? grads[0].detach().to(torch::kFloat64).contiguous() : host_f64(grads[0]);
+        const auto opts = w.options();
         Tensor g0 = torch::empty_like(w);
-        Tensor g1 = torch::empty({n, nsub}, torch::kFloat64), g2 = torch::empty({n, nsub}, torch::kFloat64);
+        Tensor g1 = torch::empty({n, nsub}, opts), g2 = torch::empty({n, nsub}, opts);
+        double *p1 = n ? g1.data_ptr<double>() : nullptr, *p2 = n ? g2.data_ptr<double>() : nullptr;
         check(h,
-              bfvmb_adjoint_backward(h, w.data_ptr<double>(), g0.data_ptr<double>(), n ? g1.data_ptr<double>() : nullptr,
-                                     n ? g2.data_ptr<double>() : nullptr),
+              dev ? bfvmb_adjoint_backward_device(h, w.data_ptr<double>(), g0.data_ptr<double>(), p1, p2)
+                  : bfvmb_adjoint_backward(h, w.data_ptr<double>(), g0.data_ptr<double>(), p1, p2),
               "adjoint_backward");
         return {g0.to(ctx->saved_data["rho_device"].toDevice()), Tensor(),
                 g1.to(ctx->saved_data["t1_device"].toDevice()), g2.to(ctx->saved_data["t2_device"].toDevice()),

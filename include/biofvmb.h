@@ -66,6 +66,11 @@ int bfvmb_set_dirichlet(bfvmb_ctx *ctx, const double *value, const uint8_t *acti
 /* Host <-> device field transfer in the reference layout (p_density_vectors). */
 int bfvmb_upload_density(bfvmb_ctx *ctx, const double *rho_voxel_major);
 int bfvmb_download_density(bfvmb_ctx *ctx, double *rho_voxel_major);
+/* The same transposes from / into a device buffer (FP64, voxel-major), in stream order on the
+ * context's stream: no host transfer and no wait (a caller on another stream synchronizes
+ * first).  For fields that GPU code produces or consumes, e.g. torch CUDA tensors. */
+int bfvmb_upload_density_device(bfvmb_ctx *ctx, const double *rho_voxel_major);
+int bfvmb_download_density_device(bfvmb_ctx *ctx, double *rho_voxel_major);
 
 /* Agents for the source/sink update: voxel index (reference numbering), per-agent
  * cell_source_sink_solver_temp1/temp2 ([n][nsub], BioFVM_basic_agent.cpp:155-176) and the
@@ -176,6 +181,11 @@ int bfvmb_autotune(bfvmb_ctx *ctx, int reps, int *chosen);
 int bfvmb_adjoint_forward(bfvmb_ctx *ctx, int nsteps, int checkpoint_every);
 int bfvmb_adjoint_backward(bfvmb_ctx *ctx, const double *seed, double *grad_rho0, double *grad_temp1,
                            double *grad_temp2);
+/* bfvmb_adjoint_backward with device pointers for the seed and the three gradients (same
+ * layouts), in stream order on the context's stream: no host transfer, no wait at the end
+ * (a caller on another stream synchronizes first). */
+int bfvmb_adjoint_backward_device(bfvmb_ctx *ctx, const double *seed, double *grad_rho0,
+                                  double *grad_temp1, double *grad_temp2);
 
 /* ---- Density gradients (Microenvironment::compute_all_gradient_vectors and
  * compute_gradient_vector, BioFVM_microenvironment.cpp:1243-1533): per voxel, substrate and

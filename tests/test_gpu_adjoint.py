@@ -358,3 +358,59 @@ def test_new_records_reuse_state_only_when_valid():
         done += 1
     s.close()
     assert done == len(plan)
+
+
+def test_device_pointers_give_the_host_results():
+    """bfvmb_upload/download_density_device and bfvmb_adjoint_backward_device (CUDA tensors,
+    no host transfer) against the host-pointer calls, bit for bit, with inactive agents."""
+    import torch
+    c = _case(nsteps=8)
+    tgt = _target(c)
+    voxel, t1, t2 = orc.agent_tables(c)
+    act = np.ones(c.nagents, np.uint8)
+    act[::4] = 0
+    print("[gpu] adjoint device pointers", flush=True)
+    outs = []
+    for dev in (False, True):
+        from biofvmb import LODSolver
+        s = LODSolver(c.nx, c.ny, c.nz, c.nsub, c.dx, "fp64")
+        s.set_coefficients(c.D, c.decay, c.dt)
+        s.set_dirichlet(*cf.effective_dirichlet(c))
+        s.set_agents(voxel, t1, t2, act)
+        if dev:
+            s.upload(torch.tensor(c.init, device="cuda:0"))
+        else:
+            s.upload(c.init)
+        s.adjoint_forward(c.nsteps, 3)
+        if dev:
+            rT = s.download(torch.empty(c.init.shape, dtype=torch.float64, device="cuda:0"))
+            g = s.adjoint_backward(2.0 * (rT - torch.tensor(tgt, device="cuda:0")))
+            outs.append([rT.cpu().numpy()] + [x.cpu().numpy() for x in g])
+        else:
+            rT = s.download()
+            outs.append([rT] + list(s.adjoint_backward(2.0 * (rT - tgt))))
+        s.close()
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+    assert np.all(outs[1][2][::4] == 0.0) and np.any(outs[1][2][1::4] != 0.0)
+
+
+def test_torch_op_cuda_and_cpu_tensors_agree():
+    import torch
+    from biofvmb import autograd as ag
+    c = _case(nsteps=6)
+    tgt = _target(c)
+    voxel, t1, t2 = orc.agent_tables(c)
+    print("[gpu] torch op cuda vs cpu tensors", flush=True)
+    res = []
+    for dev in ("cpu", "cuda"):
+        s = _solver(c)
+        r0 = torch.tensor(c.init, device=dev, requires_grad=True)
+        a1 = torch.tensor(t1, device=dev, requires_grad=True)
+        a2 = torch.tensor(t2, device=dev, requires_grad=True)
+        rT = ag.lod_run(s, r0, torch.tensor(voxel, device=dev), a1, a2, c.nsteps, 4)
+        ((rT - torch.tensor(tgt, device=dev)) ** 2).sum().backward()
+        res.append([x.detach().cpu().numpy() for x in (rT, r0.grad, a1.grad, a2.grad)])
+        s.close()
+    for a, b in zip(*res):
+        assert np.array_equal(a, b)
