@@ -148,18 +148,22 @@ __host__ __device__ __forceinline__ int x_line_store(int nx, long long budget)
 // depth); table entries are read in batches of TB per wait.  Before the line store, up to
 // reg_rows<T>() forward results per line are kept in registers (the wavefront has a SIMD to
 // itself at the default four per CU, and its register file is larger than its LDS share).
-constexpr int TB = 16;
-// Register rows per lane: 128 in FP64 (256 registers), 320 in FP32 (one register per row), so
-// that with the line store an FP32 400-long line stays on chip (measured at 400^3 x 4: y
-// 0.522 -> 0.499 ms, z 0.542 -> 0.505 ms against 128 FP32 rows).
+// Register rows per lane: 160 in FP64 (320 registers, room made by table batches of 8), 320 in
+// FP32 (one register per row, table batches of 16), so that with the line store an FP32
+// 400-long line stays on chip.  Measured at 400^3 x 4: FP32 y 0.522 -> 0.499 ms, z 0.542 ->
+// 0.505 ms against 128 FP32 rows; FP64 y 0.972 -> 0.961, z 1.029 -> 1.012 against 128 rows and
+// batches of 16.
 template <class T>
-constexpr int reg_rows() { return sizeof(T) == 8 ? 128 : 320; }
+constexpr int reg_rows() { return sizeof(T) == 8 ? 160 : 320; }
+template <class T>
+constexpr int table_batch() { return sizeof(T) == 8 ? 8 : 16; }
 
 template <class T, int AXIS, int P>
 __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict__ descs,
                                                  const long long *__restrict__ prefix, int ndom, int parity,
                                                  int budget)
 {
+    constexpr int TB = table_batch<T>();
     static_assert(P % TB == 0, "ring depth must be a multiple of the table batch");
     constexpr int RC = (P == 64 ? reg_rows<T>() / 2 : reg_rows<T>()) / P;  // rings of rows kept in registers
                                                                  // (fewer beside a 64-row ring)
