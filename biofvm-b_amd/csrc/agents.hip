@@ -32,6 +32,8 @@ struct AgentRemap {
     int *vin = nullptr, *vout = nullptr;       // [n0] compact indices
     int *head = nullptr, *sid = nullptr;       // [n0] segment heads, segment ids
     int *counts = nullptr;               // [2] segments, agents alive
+    int *seg_start_new = nullptr;        // CSR arrays with room for a segment per agent, adopted
+    long long *seg_off_new = nullptr;    // by the domain after the first successful regrouping
     void *tmp = nullptr;                 // hipcub scratch
     size_t tmp_bytes = 0;
     std::vector<int> host_ids;
@@ -45,7 +47,7 @@ void free_remap(Domain &d)
         return;
     for (void *p : {(void *)r->ids, (void *)r->alive, (void *)r->tab, (void *)r->pos, (void *)r->kin,
                     (void *)r->kout, (void *)r->vin, (void *)r->vout, (void *)r->head, (void *)r->sid,
-                    (void *)r->counts, r->tmp})
+                    (void *)r->counts, r->tmp, (void *)r->seg_start_new, (void *)r->seg_off_new})
         (void)hipFree(p);
     delete r;
     d.remap = nullptr;
@@ -176,9 +178,7 @@ int remap_setup(Group *g, Domain &d)
     std::vector<int> cidx(n0);
     for (int k = 0; k < n0; ++k)
         cidx[k] = (int)(std::lower_bound(r->host_ids.begin(), r->host_ids.end(), d.agent_order[k]) - r->host_ids.begin());
-    // the CSR may now need a segment per agent
-    int *seg_start = nullptr;
-    long long *seg_off = nullptr;
+    // the CSR may now need a segment per agent: new arrays, adopted once they are filled
     int *d_cidx = nullptr;
     int rc;
     const size_t tab = (size_t)ns * n0 * sizeof(double);
@@ -192,16 +192,10 @@ int remap_setup(Group *g, Domain &d)
         (rc = hip_check(g, hipMalloc((void **)&r->head, n0 * sizeof(int)), "agent heads")) ||
         (rc = hip_check(g, hipMalloc((void **)&r->sid, n0 * sizeof(int)), "agent segments")) ||
         (rc = hip_check(g, hipMalloc((void **)&r->counts, 2 * sizeof(int)), "agent counts")) ||
-        (rc = hip_check(g, hipMalloc((void **)&seg_start, (n0 + 1) * sizeof(int)), "agent segments")) ||
-        (rc = hip_check(g, hipMalloc((void **)&seg_off, n0 * sizeof(long long)), "agent voxels")) ||
+        (rc = hip_check(g, hipMalloc((void **)&r->seg_start_new, (n0 + 1) * sizeof(int)), "agent segments")) ||
+        (rc = hip_check(g, hipMalloc((void **)&r->seg_off_new, n0 * sizeof(long long)), "agent voxels")) ||
         (rc = hip_check(g, hipMalloc((void **)&d_cidx, n0 * sizeof(int)), "agent ranks")))
         goto fail;
-    (void)hipFree(d.seg_start);
-    (void)hipFree(d.seg_off);
-    d.seg_start = seg_start;
-    d.seg_off = seg_off;
-    seg_start = nullptr;
-    seg_off = nullptr;
     {
         size_t b1 = 0, b2 = 0;
         if ((rc = hip_check(g, hipcub::DeviceRadixSort::SortPairs(nullptr, b1, r->kin, r->kout, r->vin, r->vout, n0, 0,
@@ -227,8 +221,6 @@ int remap_setup(Group *g, Domain &d)
     (void)hipFree(d_cidx);
     return BFVMB_OK;
 fail:
-    (void)hipFree(seg_start);
-    (void)hipFree(seg_off);
     (void)hipFree(d_cidx);
     free_remap(d);
     return rc;
@@ -288,17 +280,41 @@ int domain_update_agent_positions(Group *g, int i, int n, const double *pos, con
     if ((rc = hip_check(g, hipGetLastError(), "agent heads")) ||
         (rc = hip_check(g, hipcub::DeviceScan::ExclusiveSum(r->tmp, bytes, r->head, r->sid, n0, g->stream),
                         "agent scan")) ||
-        (rc = hip_check(g, hipMemsetAsync(r->counts, 0, 2 * sizeof(int), g->stream), "agent counts")) ||
-        (rc = hip_check(g, hipMemsetAsync(d.seg_start, 0, sizeof(int), g->stream), "agent segments")))
+        (rc = hip_check(g, hipMemsetAsync(r->counts, 0, 2 * sizeof(int), g->stream), "agent counts")))
         return rc;
-    hipLaunchKernelGGL(k_agent_build, grid, block, 0, g->stream, r->kout, r->vout, r->head, r->sid, n0, d.nsub, dead,
-                       r->tab, d.seg_start, d.seg_off, d.t1, d.t2, r->counts);
+    // up to here the domain's agent list is untouched; from the build on it is rewritten, and a
+    // failure drops it (no descriptor may keep pointing at a half-built list)
+    int *seg_start = r->seg_start_new ? r->seg_start_new : d.seg_start;
+    long long *seg_off = r->seg_off_new ? r->seg_off_new : d.seg_off;
     int counts[2] = {0, 0};
-    if ((rc = hip_check(g, hipGetLastError(), "agent build")) ||
-        (rc = hip_check(g, hipMemcpyAsync(counts, r->counts, sizeof(counts), hipMemcpyDeviceToHost, g->stream),
-                        "agent counts")) ||
-        (rc = hip_check(g, hipStreamSynchronize(g->stream), "agent update")))
+    rc = hip_check(g, hipMemsetAsync(seg_start, 0, sizeof(int), g->stream), "agent segments");
+    if (!rc) {
+        hipLaunchKernelGGL(k_agent_build, grid, block, 0, g->stream, r->kout, r->vout, r->head, r->sid, n0, d.nsub,
+                           dead, r->tab, seg_start, seg_off, d.t1, d.t2, r->counts);
+        rc = hip_check(g, hipGetLastError(), "agent build");
+    }
+    if (!rc)
+        rc = hip_check(g, hipMemcpyAsync(counts, r->counts, sizeof(counts), hipMemcpyDeviceToHost, g->stream),
+                       "agent counts");
+    if (!rc)
+        rc = hip_check(g, hipStreamSynchronize(g->stream), "agent update");
+    if (rc) {
+        (void)hipStreamSynchronize(g->stream);
+        free_agents(d);
+        d.agents_given = 0;
+        d.agent_order.clear();
+        g->desc_dirty = true;
+        g->epoch++;
         return rc;
+    }
+    if (r->seg_start_new) {  // first regrouping: the new arrays replace the set_agents ones
+        (void)hipFree(d.seg_start);
+        (void)hipFree(d.seg_off);
+        d.seg_start = r->seg_start_new;
+        d.seg_off = r->seg_off_new;
+        r->seg_start_new = nullptr;
+        r->seg_off_new = nullptr;
+    }
     r->order_stale = true;  // read back when someone needs it (the adjoint's rate gradients)
     d.nseg = counts[0];
     g->desc_dirty = true;

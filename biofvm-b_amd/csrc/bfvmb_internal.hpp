@@ -157,6 +157,7 @@ int domain_densities_at(Group *g, int i, int n, const int *voxel, double *out, i
 int domain_upload_device(Group *g, int i, const double *d_rho);   // group.hip, stream-ordered
 int domain_download_device(Group *g, int i, double *d_rho);
 void free_remap(Domain &d);
+void free_agents(Domain &d);  // group.hip: releases a domain's agent list (and its remap state)
 int group_bulk(Group *g, double dt);
 int group_steps(Group *g, int nsteps, bool with_agents, bool use_graph);
 int group_set_stream(Group *g, void *stream);

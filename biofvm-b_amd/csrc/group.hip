@@ -74,18 +74,6 @@ int upload_dirichlet(Group *g, Domain &d)
     return rc;
 }
 
-void free_agents(Domain &d)
-{
-    free_remap(d);
-    (void)hipFree(d.seg_start);
-    (void)hipFree(d.seg_off);
-    (void)hipFree(d.t1);
-    (void)hipFree(d.t2);
-    d.seg_start = nullptr;
-    d.seg_off = nullptr;
-    d.t1 = d.t2 = nullptr;
-    d.nseg = d.nact = 0;
-}
 
 template <class T>
 std::vector<DomainDesc<T>> build_descs(Group *g)
@@ -123,6 +111,19 @@ std::vector<DomainDesc<T>> build_descs(Group *g)
 }
 
 }  // namespace
+
+void free_agents(Domain &d)
+{
+    free_remap(d);
+    (void)hipFree(d.seg_start);
+    (void)hipFree(d.seg_off);
+    (void)hipFree(d.t1);
+    (void)hipFree(d.t2);
+    d.seg_start = nullptr;
+    d.seg_off = nullptr;
+    d.t1 = d.t2 = nullptr;
+    d.nseg = d.nact = 0;
+}
 
 int sync_descriptors(Group *g)
 {
@@ -477,8 +478,12 @@ int domain_set_bulk(Group *g, int i, const double *supply, const double *target,
         if ((rc = hip_check(g, hipMemcpyAsync(g->stage, src[k]->data(), n * sizeof(double), hipMemcpyHostToDevice,
                                               g->stream), "bulk upload")) ||
             (rc = launch_aos_to_soa_f64(g, d, static_cast<const double *>(g->stage), d.bulk + k * n)) ||
-            (rc = hip_check(g, hipStreamSynchronize(g->stream), "bulk upload")))
+            (rc = hip_check(g, hipStreamSynchronize(g->stream), "bulk upload"))) {
+            (void)hipStreamSynchronize(g->stream);
+            (void)hipFree(d.bulk);  // no bulk phase with half-written rates
+            d.bulk = nullptr;
             return rc;
+        }
     return BFVMB_OK;
 }
 
