@@ -70,6 +70,7 @@ SIGNATURES = {
     "bfvmb_batch_run": (_c.c_int, [_vp, _c.c_int, _c.c_int]),
     "bfvmb_batch_sync": (_c.c_int, [_vp]),
     "bfvmb_batch_set_launch": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_int, _c.c_int]),
+    "bfvmb_batch_launch_phase": (_c.c_int, [_vp, _c.c_int, _c.c_int]),
     "bfvmb_batch_autotune": (_c.c_int, [_vp, _c.c_int, _ip]),
     "bfvmb_update_agent_positions": (_c.c_int, [_vp, _c.c_int, _vp, _dp, _c.c_int]),
     "bfvmb_densities_at": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _c.c_int]),

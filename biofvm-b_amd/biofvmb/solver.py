@@ -386,3 +386,6 @@ class Batch:
         out = np.zeros(9, dtype=np.int32)
         self._check(self._lib.bfvmb_batch_autotune(self._h, reps, L.iptr(out)), "autotune")
         return out.reshape(3, 3)
+
+    def launch_phase(self, phase, count=1):
+        self._check(self._lib.bfvmb_batch_launch_phase(self._h, phase, count), "launch_phase")

@@ -293,4 +293,9 @@ int bfvmb_batch_autotune(bfvmb_batch *b, int reps, int *chosen)
     return b ? group_autotune(b, reps, chosen) : BFVMB_ERR_ARG;
 }
 
+int bfvmb_batch_launch_phase(bfvmb_batch *b, int phase, int count)
+{
+    return b ? group_launch_phase(b, phase, count) : BFVMB_ERR_ARG;
+}
+
 }  // extern "C"

@@ -586,7 +586,8 @@ int group_set_launch(Group *g, int sweep, long long grid, int bpc, int ring)
 }
 
 // Autotuner: for each sweep, time the candidate launch shapes (LDS wavefronts per CU; ring
-// depth for y/z) over `reps` launches with HIP events and keep the fastest.  The fields are
+// depth for y/z; the variant and a strided grid for the x sweep) over `reps` launches with
+// HIP events and keep the fastest.  The fields are
 // saved before and restored after, so tuning does not advance the simulation.
 int group_autotune(Group *g, int reps, int *chosen)
 {
@@ -616,27 +617,35 @@ int group_autotune(Group *g, int reps, int *chosen)
     for (int axis = 0; axis < 3 && !rc; ++axis) {
         float best = 1e30f;
         LaunchShape best_shape;
-        const int rings[2] = {32, 64};
+        // y/z: rows in flight; x: the FP32 variant by its chunks in flight (FP64 has one)
+        const int yz_rings[2] = {32, 64}, x_rings[3] = {1, 2, 3};
+        const int *rings = axis == 0 ? x_rings : yz_rings;
+        const int nring = axis == 0 ? (g->elem == 4 ? 3 : 1) : 2;
         const int bpcs[5] = {2, 3, 4, 6, 8};
+        // x also tries a grid of one block per wavefront slot (up to four per CU) striding
+        // over the tiles: the short lines of a batch run a few percent faster that way
+        const int ngrid = axis == 0 ? 2 : 1;
         for (int b = 0; b < 5 && !rc; ++b)
-            for (int r = 0; r < (axis == 0 ? 1 : 2) && !rc; ++r) {
-                g->launch[axis] = LaunchShape();
-                g->launch[axis].lds_blocks_per_cu = bpcs[b];
-                g->launch[axis].ring = axis == 0 ? 0 : rings[r];
-                if ((rc = launch_sweep(g, axis)))  // warm-up
-                    break;
-                (void)hipEventRecord(e0, g->stream);
-                for (int k = 0; k < reps && !rc; ++k)
-                    rc = launch_sweep(g, axis);
-                (void)hipEventRecord(e1, g->stream);
-                if (!rc)
-                    rc = hip_check(g, hipEventSynchronize(e1), "autotune timing");
-                float ms = 0.f;
-                if (!rc && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best) {
-                    best = ms;
-                    best_shape = g->launch[axis];
+            for (int r = 0; r < nring && !rc; ++r)
+                for (int gi = 0; gi < ngrid && !rc; ++gi) {
+                    g->launch[axis] = LaunchShape();
+                    g->launch[axis].grid = gi == 0 ? 0 : (long long)std::max(g->cu_count, 1) * std::min(bpcs[b], 4);
+                    g->launch[axis].lds_blocks_per_cu = bpcs[b];
+                    g->launch[axis].ring = axis == 0 && g->elem == 8 ? 0 : rings[r];
+                    if ((rc = launch_sweep(g, axis)))  // warm-up
+                        break;
+                    (void)hipEventRecord(e0, g->stream);
+                    for (int k = 0; k < reps && !rc; ++k)
+                        rc = launch_sweep(g, axis);
+                    (void)hipEventRecord(e1, g->stream);
+                    if (!rc)
+                        rc = hip_check(g, hipEventSynchronize(e1), "autotune timing");
+                    float ms = 0.f;
+                    if (!rc && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best) {
+                        best = ms;
+                        best_shape = g->launch[axis];
+                    }
                 }
-            }
         g->launch[axis] = best_shape;
         if (chosen) {
             chosen[3 * axis] = best_shape.lds_blocks_per_cu;

@@ -55,6 +55,35 @@ __device__ __forceinline__ gmem<T> as_global(T *p)
     return (gmem<T>)p;
 }
 
+// Checkpoint of the forward recurrence at a head-segment end, kept in output elements that
+// the backward pass overwrites later.  FP64 storage keeps the value itself; FP32 storage
+// keeps the double's two 32-bit halves in two elements (off and off2), so that a recomputed
+// segment starts from exactly the value of the first pass, and results do not depend on how
+// much of a line stays on chip (launch shape, kernel variant).
+template <class T>
+__device__ __forceinline__ void put_ck(gmem<T> p, size_t off, size_t off2, double y)
+{
+    if constexpr (sizeof(T) == 8) {
+        (void)off2;
+        p[off] = y;
+    } else {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(y);
+        p[off] = __uint_as_float((unsigned)b);
+        p[off2] = __uint_as_float((unsigned)(b >> 32));
+    }
+}
+template <class T>
+__device__ __forceinline__ double get_ck(gmem<T> p, size_t off, size_t off2)
+{
+    if constexpr (sizeof(T) == 8) {
+        (void)off2;
+        return p[off];
+    } else {
+        const unsigned long long lo = __float_as_uint(p[off]), hi = __float_as_uint(p[off2]);
+        return __longlong_as_double((long long)((hi << 32) | lo));
+    }
+}
+
 // a/d correctly rounded from r = RN(1/d): q = a*r, two corrections (Markstein).  Valid for
 // |a| >= 2^-960 (no underflow in the corrections); callers flag smaller numerators.
 __device__ __forceinline__ double div_corr(double a, double d, double r)
@@ -194,18 +223,6 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
         const long long lt = tile - prefix[di];
         const int s = __builtin_amdgcn_readfirstlane((int)(lt / tiles_per_s));  // 64-bit division runs on
                                                                               // the vector unit
-        if (di != tab_dom || s != tab_s) {  // stage this substrate's Thomas tables
-            __syncthreads();
-            const double *gd = D.tab[AXIS].den + (size_t)s * n, *gr = D.tab[AXIS].rden + (size_t)s * n;
-            const double *gc = D.tab[AXIS].cc + (size_t)s * n;
-            for (int i = lane; i < n; i += kWave) {
-                tab2[i] = make_double2(gd[i], gr[i]);
-                tabc[i] = gc[i];
-            }
-            __syncthreads();
-            tab_dom = di;
-            tab_s = s;
-        }
         long long line = (lt - (long long)s * tiles_per_s) * kWave + lane;
         if (line >= lines_per_s)
             line = lines_per_s - 1;  // idle lanes repeat the last line (same values, same stores)
@@ -231,6 +248,20 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
 #pragma unroll
         for (int j = 0; j < P; ++j)
             buf[j] = src[(size_t)min(j, n - 1) * stride];
+        // stage this substrate's Thomas tables after the first ring of rows is on its way, so
+        // that the two load latencies overlap (a block starts with both at every new tile)
+        if (di != tab_dom || s != tab_s) {
+            __syncthreads();
+            const double *gd = D.tab[AXIS].den + (size_t)s * n, *gr = D.tab[AXIS].rden + (size_t)s * n;
+            const double *gc = D.tab[AXIS].cc + (size_t)s * n;
+            for (int i = lane; i < n; i += kWave) {
+                tab2[i] = make_double2(gd[i], gr[i]);
+                tabc[i] = gc[i];
+            }
+            __syncthreads();
+            tab_dom = di;
+            tab_s = s;
+        }
         double y = -0.0;
         unsigned amin = 0xffffffffu;
         T regs[RC * P];  // register rows [lo, lds_lo)
@@ -273,7 +304,7 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
         for (int next_ck = r0; k0 < lo; k0 += P) {  // head: checkpoints only
             chunk(k0, 0, n - 1, no_store, inexact, pf, nullptr);
             if (k0 + P == next_ck && next_ck < lo) {
-                dst[(size_t)(next_ck - 1) * stride] = (T)y;
+                put_ck<T>(dst, (size_t)(next_ck - 1) * stride, (size_t)(next_ck - 2) * stride, y);
                 next_ck += S;
             }
         }
@@ -362,7 +393,7 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
         auto seg_begin = [&](int b) { return b > r0 ? b - S : 0; };
         auto prefetch_segment = [&](int b) {
             const int a = seg_begin(b);
-            yck = a == 0 ? -0.0 : (double)dst[(size_t)(a - 1) * stride];
+            yck = a == 0 ? -0.0 : get_ck<T>(dst, (size_t)(a - 1) * stride, (size_t)(a - 2) * stride);
 #pragma unroll
             for (int j = 0; j < P; ++j)
                 buf[j] = src[(size_t)min(a + j, b - 1) * stride];
@@ -425,21 +456,40 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
 //    head and the line store (an FP32 chunk takes 16 registers), so that with the line
 //    store's 8 chunks a 400-long line is recomputed nowhere.  Measured at 400^3 x 4: deeper
 //    rings with fewer register chunks were slower at every step (3 and 15: -10%).
-//  * FP32 at more wavefronts per CU (3, 0): two wavefronts per SIMD and no register chunks,
-//    the faster choice for short lines that the line store holds whole.
+//  * FP32 (2, 16): as (1, 17) with a second chunk in flight in place of one register chunk
+//    (lines up to 384 columns stay on chip).
+//  * FP32 (3, 0): two wavefronts per SIMD and no register chunks, three chunks in flight (the
+//    kernel needs ~230 registers per lane without register chunks, so two wavefronts per SIMD
+//    leave no room for them: 8 register chunks spilled 392 bytes per lane).
+// The launch shape's ring value picks the FP32 variant by its chunks in flight (1, 2 or 3);
+// ring 0 takes (1, 17) at up to four wavefronts per CU and (3, 0) above.
 struct XVariant {
     int pc, regc;
 };
 template <class T>
-XVariant x_variant(int bpc)
+XVariant x_variant(int bpc, int ring)
 {
     if (sizeof(T) == 8)
         return {2, 0};
+    if (ring == 1)
+        return {1, 17};
+    if (ring == 2)
+        return {2, 16};
+    if (ring == 3)
+        return {3, 0};
     return bpc <= 4 ? XVariant{1, 17} : XVariant{3, 0};
 }
 
+// Wavefronts per SIMD an x-sweep variant is compiled for: two for FP64 without register
+// chunks (256 registers per lane), otherwise the compiler's choice.
+template <class T, int REGC>
+constexpr int x_waves_per_simd()
+{
+    return sizeof(T) == 8 && REGC == 0 ? 2 : 1;
+}
+
 template <class T, int PC, int REGC>
-__global__ __launch_bounds__(64, (sizeof(T) == 8 && REGC == 0) ? 2 : 1) void k_sweep_x(const DomainDesc<T> *__restrict__ descs,
+__global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(const DomainDesc<T> *__restrict__ descs,
                                                 const long long *__restrict__ prefix, int ndom, int parity,
                                                 int budget)
 {
@@ -554,6 +604,11 @@ __global__ __launch_bounds__(64, (sizeof(T) == 8 && REGC == 0) ? 2 : 1) void k_s
         auto ck_off = [&](int ca) -> size_t {
             return full_rows ? (size_t)(lane / W) * nx + (size_t)ca * W + (lane % W) : my_off + (size_t)ca * W - 1;
         };
+        // second element of an FP32 checkpoint: the next RPI rows of chunk ca, or the column
+        // before ck_off's
+        auto ck_off2 = [&](int ca) -> size_t {
+            return full_rows ? ck_off(ca) + (size_t)RPI * nx : my_off + (size_t)ca * W - 2;
+        };
         T ring[PC][NI];
         T regs[REGC > 0 ? REGC * W : 1];  // forward results of chunks [cr, cl)
         double y = -0.0;
@@ -622,7 +677,7 @@ __global__ __launch_bounds__(64, (sizeof(T) == 8 && REGC == 0) ? 2 : 1) void k_s
                             else
                                 fwd_chunk(c, ring[p], min(c + PC, nch - 1), cl, store, exact_tag, amin, nullptr);
                             if (c + 1 == next_ck && next_ck < cr) {  // checkpoint at a segment end
-                                dst[ck_off(next_ck)] = (T)y;
+                                put_ck<T>(dst, ck_off(next_ck), ck_off2(next_ck), y);
                                 next_ck += Sc;
                             }
                         }
@@ -635,7 +690,7 @@ __global__ __launch_bounds__(64, (sizeof(T) == 8 && REGC == 0) ? 2 : 1) void k_s
                         const int c = c0 + p;
                         fwd_chunk(c, ring[p], min(c + PC, nch - 1), 0, no_store, exact_tag, amin, nullptr);
                         if (c + 1 == next_ck && next_ck < cr) {  // checkpoint at a segment end
-                            dst[ck_off(next_ck)] = (T)y;
+                            put_ck<T>(dst, ck_off(next_ck), ck_off2(next_ck), y);
                             next_ck += Sc;
                         }
                     }
@@ -709,7 +764,7 @@ __global__ __launch_bounds__(64, (sizeof(T) == 8 && REGC == 0) ? 2 : 1) void k_s
         auto seg_begin = [&](int cb) { return cb > r0c ? cb - Sc : 0; };
         auto prefetch_segment = [&](int cb) {
             const int ca = seg_begin(cb);
-            yck = ca == 0 ? -0.0 : (double)dst[ck_off(ca)];
+            yck = ca == 0 ? -0.0 : get_ck<T>(dst, ck_off(ca), ck_off2(ca));
 #pragma unroll
             for (int p = 0; p < PC; ++p)
                 load_chunk(src, min(ca + p, cb - 1), ring[p]);
@@ -757,7 +812,7 @@ constexpr size_t kLdsBytes = 160 * 1024;
 struct Shape {
     long long grid;  // 0 = one block per tile
     int bpc;         // sweep wavefronts sharing one CU's LDS
-    int ring;        // y/z: rows in flight per lane
+    int ring;        // y/z: rows in flight per lane; x: chunks in flight (0 = by bpc)
 };
 
 Shape shape_of(const Group *g, int axis)
@@ -770,7 +825,7 @@ Shape shape_of(const Group *g, int axis)
     // four FP32 x sweep wavefronts (one per SIMD, holding register chunks), and y/z keep 32
     // rows per lane in flight
     sh.bpc = u.lds_blocks_per_cu > 0 ? u.lds_blocks_per_cu : (axis == 0 && g->elem == 8 ? 6 : 4);
-    sh.ring = u.ring > 0 ? u.ring : 32;
+    sh.ring = u.ring > 0 ? u.ring : (axis == 0 ? 0 : 32);  // x: the variant's chunks in flight
     return sh;
 }
 
@@ -793,8 +848,10 @@ void launch_x(const Group *g, const DomainDesc<T> *d, const long long *pre, int 
     if constexpr (sizeof(T) == 8) {
         (void)xv;
         hipLaunchKernelGGL((k_sweep_x<T, 2, 0>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
-    } else if (xv.regc > 0) {
+    } else if (xv.pc == 1) {
         hipLaunchKernelGGL((k_sweep_x<T, 1, 17>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+    } else if (xv.pc == 2) {
+        hipLaunchKernelGGL((k_sweep_x<T, 2, 16>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
     } else {
         hipLaunchKernelGGL((k_sweep_x<T, 3, 0>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
     }
@@ -808,7 +865,7 @@ int launch_sweep_t(Group *g, int axis)
         return rc;
     const int ndom = (int)g->doms.size();
     const Shape sh = shape_of(g, axis);
-    const XVariant xv = x_variant<T>(sh.bpc);
+    const XVariant xv = x_variant<T>(sh.bpc, axis == 0 ? sh.ring : 0);
     size_t budget = std::min(kLdsBytes / sh.bpc, g->max_lds_per_block);
     // LDS actually needed by the largest domain, capped by the budget; the budget is raised
     // (fewer wavefronts per CU) where a line store would hold less than one 16-row ring or
@@ -914,6 +971,8 @@ int sweep_lds_attributes(size_t lds_bytes)
         e = hipFuncSetAttribute((const void *)k_sweep_x<double, 2, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e == hipSuccess)
         e = hipFuncSetAttribute((const void *)k_sweep_x<float, 1, 17>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)k_sweep_x<float, 2, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e == hipSuccess)
         e = hipFuncSetAttribute((const void *)k_sweep_x<float, 3, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     return e == hipSuccess ? 0 : (int)e;

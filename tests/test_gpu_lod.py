@@ -125,11 +125,16 @@ def test_set_agents_with_temps_and_inactive():
 @pytest.mark.parametrize("name", ["chunks_130x70x33x2", "long_x_700x5x6x2", "long_z_5x6x700x2"])
 @pytest.mark.parametrize("launch", [(7, 1, 16), (0, 2, 32), (3, 8, 64), (0, 4, 16)])
 def test_launch_shapes_do_not_change_results(name, launch):
-    """Strided grids, ring depths and smaller LDS line stores (more of each line parked in
-    the output buffer) give the same bits as the default launch."""
+    """Strided grids, ring depths and smaller LDS line stores (more of each line recomputed
+    from checkpoints) give the same bits as the default launch: FP64 the oracle's, FP32 the
+    default FP32 launch's (within 1e-5 of the oracle)."""
     case = _case(name)
     print(f"[gpu] launch {launch} {name}", flush=True)
     ref = orc.run(case)
+    s = _solver(case, "fp32")
+    s.run(case.nsteps)
+    base32 = s.download()
+    s.close()
     for prec in ("fp64", "fp32"):
         s = _solver(case, prec)
         grid, bpc, ring = launch
@@ -141,6 +146,31 @@ def test_launch_shapes_do_not_change_results(name, launch):
             assert np.array_equal(got, ref)
         else:
             assert rel_err(got, ref)[0] <= 1e-5
+            assert np.array_equal(got, base32)
+
+
+@pytest.mark.parametrize("name", ["chunks_130x70x33x2", "long_x_700x5x6x2"])
+def test_fp32_x_variants_agree_bitwise(name):
+    """The FP32 x-sweep variants (1, 2 or 3 column chunks in flight, with 17, 16 or no
+    register chunks) at several LDS budgets and strided grids: each stores the same
+    FP32-rounded forward results, so every launch gives the default launch's bits, within
+    1e-5 of the FP64 oracle."""
+    case = _case(name)
+    print(f"[gpu] fp32 x variants {name}", flush=True)
+    ref = orc.run(case)
+    s = _solver(case, "fp32")
+    s.run(case.nsteps)
+    base = s.download()
+    s.close()
+    assert rel_err(base, ref)[0] <= 1e-5
+    for ring in (1, 2, 3):
+        for grid, bpc in ((0, 2), (5, 4), (0, 8)):
+            s = _solver(case, "fp32")
+            s.set_launch(0, grid, bpc, ring)
+            s.run(case.nsteps)
+            got = s.download()
+            s.close()
+            assert np.array_equal(got, base), (ring, grid, bpc)
 
 
 def _tiny_case(nx, ny, nz, nsub, seed, mixed):
