@@ -618,9 +618,9 @@ int group_autotune(Group *g, int reps, int *chosen)
         float best = 1e30f;
         LaunchShape best_shape;
         // y/z: rows in flight; x: the FP32 variant by its chunks in flight (FP64 has one)
-        const int yz_rings[2] = {32, 64}, x_rings[3] = {1, 2, 3};
+        const int yz_rings[2] = {32, 64}, x_rings[4] = {1, 2, 3, 4};
         const int *rings = axis == 0 ? x_rings : yz_rings;
-        const int nring = axis == 0 ? (g->elem == 4 ? 3 : 1) : 2;
+        const int nring = axis == 0 ? (g->elem == 4 ? 4 : 1) : 2;
         const int bpcs[5] = {2, 3, 4, 6, 8};
         // x also tries a grid of one block per wavefront slot (up to four per CU) striding
         // over the tiles: the short lines of a batch run a few percent faster that way

@@ -157,11 +157,13 @@ template <class T>
 constexpr int x_chunk() { return 16; }
 
 // Columns per row kept in the x line store (a multiple of the chunk width, odd pitch),
-// after the 64 x (W+1) staging tile.
+// after the 64 x (W+1) staging tile and, with tables in LDS (tl), two FP64 tables of nx.
 template <class T>
-__host__ __device__ __forceinline__ int x_line_store(int nx, long long budget)
+__host__ __device__ __forceinline__ int x_line_store(int nx, long long budget, bool tl)
 {
     constexpr int W = x_chunk<T>();
+    if (tl)
+        budget -= 16LL * nx;
     const long long stg = (long long)kWave * (W + 1) * sizeof(T);
     long long L = budget > stg ? (budget - stg) / (kWave * (long long)sizeof(T)) - 1 : 0;
     L = L / W * W;
@@ -456,53 +458,64 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
 //    head and the line store (an FP32 chunk takes 16 registers), so that with the line
 //    store's 8 chunks a 400-long line is recomputed nowhere.  Measured at 400^3 x 4: deeper
 //    rings with fewer register chunks were slower at every step (3 and 15: -10%).
-//  * FP32 (2, 16): as (1, 17) with a second chunk in flight in place of one register chunk
-//    (lines up to 384 columns stay on chip).
 //  * FP32 (3, 0): two wavefronts per SIMD and no register chunks, three chunks in flight (the
 //    kernel needs ~230 registers per lane without register chunks, so two wavefronts per SIMD
-//    leave no room for them: 8 register chunks spilled 392 bytes per lane).
-// The launch shape's ring value picks the FP32 variant by its chunks in flight (1, 2 or 3);
-// ring 0 takes (1, 17) at up to four wavefronts per CU and (3, 0) above.
+//    leave no room for them: 8 register chunks spilled 392 bytes per lane; a (2, 16) variant,
+//    two chunks in flight beside 16 register chunks, measured no faster than (1, 17)).
+// Each FP32 variant reads its Thomas tables (rden forward, cc backward) either with scalar
+// loads or, staged once per tile, from LDS (tl): scalar loads return out of order, so every
+// wait for an LDS access also waits for them, and the FP32 sweep, one wavefront per SIMD,
+// has no other wavefront to cover that; the LDS copy costs 16 bytes per column of line store.
+// The launch shape's ring value picks the FP32 variant: 1 = (1, 17), 2 = (1, 17) tl,
+// 3 = (3, 0), 4 = (3, 0) tl; ring 0 takes 1 at up to four wavefronts per CU and 3 above.
+// Measured: at 400^3 x 4, 1 and 2 run equally fast (0.652 / 0.656 ms: the tables cost a
+// chunk of line store, and a 400-column line then recomputes one chunk); on the batch of 64
+// uneven domains (lines of 32-256) 2 takes 2.41 ms against 2.65 ms for 1.
 struct XVariant {
     int pc, regc;
+    bool tl;
 };
 template <class T>
 XVariant x_variant(int bpc, int ring)
 {
     if (sizeof(T) == 8)
-        return {2, 0};
+        return {2, 0, false};
     if (ring == 1)
-        return {1, 17};
+        return {1, 17, false};
     if (ring == 2)
-        return {2, 16};
+        return {1, 17, true};
     if (ring == 3)
-        return {3, 0};
-    return bpc <= 4 ? XVariant{1, 17} : XVariant{3, 0};
+        return {3, 0, false};
+    if (ring == 4)
+        return {3, 0, true};
+    return bpc <= 4 ? XVariant{1, 17, false} : XVariant{3, 0, false};
 }
 
-// Wavefronts per SIMD an x-sweep variant is compiled for: two for FP64 without register
-// chunks (256 registers per lane), otherwise the compiler's choice.
+// Wavefronts per SIMD an x-sweep variant is compiled for: two without register chunks (256
+// registers per lane), otherwise one.
 template <class T, int REGC>
 constexpr int x_waves_per_simd()
 {
-    return sizeof(T) == 8 && REGC == 0 ? 2 : 1;
+    return REGC == 0 ? 2 : 1;
 }
 
-template <class T, int PC, int REGC>
+template <class T, int PC, int REGC, bool TL>
 __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(const DomainDesc<T> *__restrict__ descs,
                                                 const long long *__restrict__ prefix, int ndom, int parity,
                                                 int budget)
 {
     static_assert(REGC % PC == 0, "register chunks come in whole rings");
+    static_assert(!TL || sizeof(T) == 4, "tables in LDS: FP32 variants only");
     constexpr int W = x_chunk<T>();  // columns per chunk
     constexpr int RPI = kWave / W;   // rows per load instruction (4)
     constexpr int NI = kWave / RPI;  // load instructions per chunk (W)
     extern __shared__ unsigned char smem[];
-    T *stg = reinterpret_cast<T *>(smem);  // [64][W+1]
-    T *ls = stg + kWave * (W + 1);          // [64][L+1]
+    // LDS: TL tables rden[nx], cc[nx]; staging tile [64][W+1]; line store [64][L+1]
+    double *xtr = reinterpret_cast<double *>(smem);
     const int lane = threadIdx.x;
     const int lrow = lane / W, lcol = lane % W;  // position of this lane in a load instruction
     const long long ntiles = prefix[ndom];
+    int tab_dom = -1, tab_s = -1;
 
     for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int di = __builtin_amdgcn_readfirstlane(find_domain(prefix, ndom, t));  // uniform
@@ -511,7 +524,10 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
         const size_t nvox = (size_t)nx * ny * nz;
         const int nrows = ny * nz;
         const int tiles_per_s = (nrows + kWave - 1) / kWave;
-        const int L = x_line_store<T>(nx, budget);
+        double *xtc = xtr + nx;
+        T *stg = reinterpret_cast<T *>(smem + (TL ? 16 * (size_t)nx : 0));  // [64][W+1]
+        T *ls = stg + kWave * (W + 1);                                       // [64][L+1]
+        const int L = x_line_store<T>(nx, budget, TL);
         const int Lp = L + 1;
         const int nch = (nx + W - 1) / W;
         const int Lc = L / W;  // chunks the line store holds (L is a multiple of W)
@@ -627,8 +643,13 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
 #pragma unroll
             for (int j = 0; j < W; ++j) {
                 const int kk = min(c * W + j, nx - 1);
-                dd[j] = den[kk];
-                rd[j] = rden[kk];
+                if constexpr (TL) {
+                    dd[j] = 1.0;  // unused: FP32 storage multiplies by rden
+                    rd[j] = xtr[kk];
+                } else {
+                    dd[j] = den[kk];
+                    rd[j] = rden[kk];
+                }
                 v[j] = (double)stg[lane * (W + 1) + j];
             }
             T *tgt = ls + lane * Lp + (c - base) * W;
@@ -664,6 +685,21 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
 #pragma unroll
             for (int p = 0; p < PC; ++p)
                 load_chunk(src, min(p, nch - 1), ring[p]);
+            if constexpr (TL) {
+                // this substrate's tables, staged after the first chunks are on their way so
+                // that the two load latencies overlap (FP32 only: forward runs once per tile)
+                if (di != tab_dom || s != tab_s) {
+                    __syncthreads();
+                    const double *gr = D.tab[0].rden + (size_t)s * nx, *gc = D.tab[0].cc + (size_t)s * nx;
+                    for (int i = lane; i < nx; i += kWave) {
+                        xtr[i] = gr[i];
+                        xtc[i] = gc[i];
+                    }
+                    __syncthreads();
+                    tab_dom = di;
+                    tab_s = s;
+                }
+            }
             y = -0.0;
             int next_ck = r0c;
             if constexpr (REGC == 0) {
@@ -731,7 +767,7 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
             double cw[W], v[W];
 #pragma unroll
             for (int j = 0; j < W; ++j) {
-                cw[j] = cc[min(c * W + j, nx - 1)];
+                cw[j] = TL ? xtc[min(c * W + j, nx - 1)] : cc[min(c * W + j, nx - 1)];
                 if constexpr (FROM_REGS)
                     v[j] = (double)rp[j];
                 else
@@ -847,13 +883,17 @@ void launch_x(const Group *g, const DomainDesc<T> *d, const long long *pre, int 
 {
     if constexpr (sizeof(T) == 8) {
         (void)xv;
-        hipLaunchKernelGGL((k_sweep_x<T, 2, 0>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+        hipLaunchKernelGGL((k_sweep_x<T, 2, 0, false>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
     } else if (xv.pc == 1) {
-        hipLaunchKernelGGL((k_sweep_x<T, 1, 17>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
-    } else if (xv.pc == 2) {
-        hipLaunchKernelGGL((k_sweep_x<T, 2, 16>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+        if (xv.tl)
+            hipLaunchKernelGGL((k_sweep_x<T, 1, 17, true>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+        else
+            hipLaunchKernelGGL((k_sweep_x<T, 1, 17, false>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
     } else {
-        hipLaunchKernelGGL((k_sweep_x<T, 3, 0>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+        if (xv.tl)
+            hipLaunchKernelGGL((k_sweep_x<T, 3, 0, true>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+        else
+            hipLaunchKernelGGL((k_sweep_x<T, 3, 0, false>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
     }
 }
 
@@ -865,7 +905,15 @@ int launch_sweep_t(Group *g, int axis)
         return rc;
     const int ndom = (int)g->doms.size();
     const Shape sh = shape_of(g, axis);
-    const XVariant xv = x_variant<T>(sh.bpc, axis == 0 ? sh.ring : 0);
+    XVariant xv = x_variant<T>(sh.bpc, axis == 0 ? sh.ring : 0);
+    if (axis == 0 && xv.tl) {
+        // tables in LDS only where every domain's tables fit beside the smallest line store
+        constexpr int W = x_chunk<T>();
+        for (const Domain &d : g->doms)
+            if (16 * (size_t)d.nx + (size_t)kWave * (W + 1) * sizeof(T) +
+                    (size_t)kWave * ((size_t)xv.pc * W + 1) * sizeof(T) > g->max_lds_per_block)
+                xv.tl = false;
+    }
     size_t budget = std::min(kLdsBytes / sh.bpc, g->max_lds_per_block);
     // LDS actually needed by the largest domain, capped by the budget; the budget is raised
     // (fewer wavefronts per CU) where a line store would hold less than one 16-row ring or
@@ -875,7 +923,7 @@ int launch_sweep_t(Group *g, int axis)
         size_t full, floor;
         if (axis == 0) {
             constexpr int W = x_chunk<T>();
-            const size_t stg = (size_t)kWave * (W + 1) * sizeof(T);
+            const size_t stg = (size_t)kWave * (W + 1) * sizeof(T) + (xv.tl ? 16 * (size_t)d.nx : 0);
             full = stg + (size_t)kWave * ((d.nx + W - 1) / W * W + 1) * sizeof(T);
             // one chunk, or one ring of chunks with register chunks
             const size_t chunks = xv.regc > 0 ? xv.pc : 1;
@@ -895,7 +943,7 @@ int launch_sweep_t(Group *g, int axis)
     if (axis == 0) {
         constexpr int W = x_chunk<T>();
         for (const Domain &d : g->doms) {
-            const int L = x_line_store<T>(d.nx, (long long)lds);
+            const int L = x_line_store<T>(d.nx, (long long)lds, xv.tl);
             if (L < W || (xv.regc > 0 && (d.nx + W - 1) / W > L / W && L / W < xv.pc))
                 return set_error(g, BFVMB_ERR_ARG, "sweep: LDS budget too small for the x line store");
         }
@@ -968,13 +1016,15 @@ int sweep_lds_attributes(size_t lds_bytes)
     if (e == hipSuccess)
         e = yz_attributes<float, 2>(bytes);
     if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_sweep_x<double, 2, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        e = hipFuncSetAttribute((const void *)k_sweep_x<double, 2, 0, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_sweep_x<float, 1, 17>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        e = hipFuncSetAttribute((const void *)k_sweep_x<float, 1, 17, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_sweep_x<float, 2, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        e = hipFuncSetAttribute((const void *)k_sweep_x<float, 1, 17, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_sweep_x<float, 3, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        e = hipFuncSetAttribute((const void *)k_sweep_x<float, 3, 0, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)k_sweep_x<float, 3, 0, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     return e == hipSuccess ? 0 : (int)e;
 }
 

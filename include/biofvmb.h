@@ -154,13 +154,16 @@ int bfvmb_launch_phase(bfvmb_ctx *ctx, int phase, int count);
 /* Launch shape of sweep `sweep` (0 = x, 1 = y, 2 = z, -1 = all three); 0 keeps the
  * library default: grid size (a smaller grid strides over the line tiles), how many sweep
  * wavefronts share one CU's LDS line store, and rows in flight per lane (y/z: 16/32/64) or
- * column chunks in flight (x, FP32: 1, 2 or 3, each a kernel variant with its own split of
- * registers).  Used by the autotuner; results do not depend on it. */
+ * the x-sweep kernel variant (FP32: 1 = one column chunk in flight and 17 register chunks,
+ * 3 = three chunks in flight and none, each with its Thomas tables read by scalar loads;
+ * 2 and 4 = the same with the tables staged in LDS).  Used by the autotuner; results do
+ * not depend on it. */
 int bfvmb_set_launch(bfvmb_ctx *ctx, int sweep, int grid_blocks, int lds_blocks_per_cu, int ring_depth);
 
 /* Autotuner: times the candidate launch shapes of each sweep (LDS wavefronts per CU 2, 3,
- * 4, 6, 8; ring depth 32/64 for y/z; the FP32 x variants) over `reps` launches with HIP
- * events and keeps the fastest.
+ * 4, 6, 8; ring depth 32/64 for y/z; the FP32 x variants and, for x, a grid of one block
+ * per wavefront slot striding over the tiles) over `reps` launches with HIP events and
+ * keeps the fastest.
  * The field is saved and restored, so tuning does not advance the simulation.  `chosen`
  * (may be NULL) receives 3 x {wavefronts per CU, ring, microseconds per launch}. */
 int bfvmb_autotune(bfvmb_ctx *ctx, int reps, int *chosen);

@@ -151,10 +151,10 @@ def test_launch_shapes_do_not_change_results(name, launch):
 
 @pytest.mark.parametrize("name", ["chunks_130x70x33x2", "long_x_700x5x6x2"])
 def test_fp32_x_variants_agree_bitwise(name):
-    """The FP32 x-sweep variants (1, 2 or 3 column chunks in flight, with 17, 16 or no
-    register chunks) at several LDS budgets and strided grids: each stores the same
-    FP32-rounded forward results, so every launch gives the default launch's bits, within
-    1e-5 of the FP64 oracle."""
+    """The FP32 x-sweep variants (1 or 3 column chunks in flight, with 17 or no register
+    chunks; Thomas tables read with scalar loads or staged in LDS) at several LDS budgets and
+    strided grids: each stores the same FP32-rounded forward results, so every launch gives
+    the default launch's bits, within 1e-5 of the FP64 oracle."""
     case = _case(name)
     print(f"[gpu] fp32 x variants {name}", flush=True)
     ref = orc.run(case)
@@ -163,7 +163,7 @@ def test_fp32_x_variants_agree_bitwise(name):
     base = s.download()
     s.close()
     assert rel_err(base, ref)[0] <= 1e-5
-    for ring in (1, 2, 3):
+    for ring in (1, 2, 3, 4):
         for grid, bpc in ((0, 2), (5, 4), (0, 8)):
             s = _solver(case, "fp32")
             s.set_launch(0, grid, bpc, ring)
