@@ -138,9 +138,10 @@ def bench_batched(torch, stream, steps=20, seed=3):
         e.record()
         torch.cuda.synchronize()
         sec = a.elapsed_time(e) / 1e3
+        phases = time_phases(b, torch, count=3)  # the tuned shapes, one phase at a time
         b.close()
         out[prec] = {"value": total_vs * steps / sec, "ms_per_step": sec * 1e3 / steps,
-                     "autotune": np.asarray(chosen).tolist()}
+                     "autotune": np.asarray(chosen).tolist(), "phases_ms": phases}
         log(f"batched {prec}: {total_vs * steps / sec:.4e} vs/s, {sec * 1e3 / steps:.3f} ms/step")
     return out
 
@@ -298,6 +299,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and adjoint workloads")
     ap.add_argument("--c5", action="store_true", help="also time the FP32 640^3 x 8 workload")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="time only the batched workload and print its JSON object (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device copy (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -315,6 +318,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    if args.batched_only:
+        print(json.dumps(bench_batched(torch, stream)), flush=True)
+        return
     log("generating inputs ...")
     inputs = make_inputs()
 

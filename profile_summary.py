@@ -1,6 +1,8 @@
 """Summaries of rocprofv3 runs of bench.py for profiles/ (reads the rocpd databases).
 
     python profile_summary.py trace <run_results.db>              -> kernel table (markdown)
+    python profile_summary.py trace-last <run_results.db> <n>     -> the same over the last n
+        dispatches of each kernel (after an autotuner's trial launches), with grid and LDS
     python profile_summary.py traffic <fetch.db> <write.db> <out.json>
         FETCH_SIZE / WRITE_SIZE per dispatch of each sweep kernel, scaled by the ratio of the
         true bytes of the 1 GiB calibration copy (bench.py --copy-calibration) to what the
@@ -25,6 +27,23 @@ def trace(db):
     print("|---|---|---|---|---|")
     for r in c.execute("select name,total_calls,total_duration,average,percentage from top_kernels"):
         print(f"| `{short(r[0])}` | {r[1]} | {r[2]:.1f} | {r[3]:.1f} | {r[4]:.1f} |")
+
+
+def trace_last(db, n):
+    c = sqlite3.connect(db)
+    rows = defaultdict(list)
+    for name, start, dur, grid, lds in c.execute(
+            "select name, start, duration, grid_x, lds_size from kernels order by start"):
+        rows[short(name)].append((dur / 1e3, grid, lds))
+    print(f"| kernel | calls (last) | average (us) | min (us) | max (us) | grid (threads) | LDS (bytes) |")
+    print("|---|---|---|---|---|---|---|")
+    for k, v in sorted(rows.items(), key=lambda kv: -sum(d for d, _, _ in kv[1][-n:])):
+        t = v[-n:]
+        d = [x[0] for x in t]
+        shapes = sorted({(g, l) for _, g, l in t})
+        g = ", ".join(str(x[0]) for x in shapes)
+        l = ", ".join(str(x[1]) for x in shapes)
+        print(f"| `{k}` | {len(t)} | {sum(d) / len(d):.1f} | {min(d):.1f} | {max(d):.1f} | {g} | {l} |")
 
 
 def counter(db, name):
@@ -66,5 +85,7 @@ def traffic(fetch_db, write_db, out):
 if __name__ == "__main__":
     if sys.argv[1] == "trace":
         trace(sys.argv[2])
+    elif sys.argv[1] == "trace-last":
+        trace_last(sys.argv[2], int(sys.argv[3]))
     else:
         traffic(sys.argv[2], sys.argv[3], sys.argv[4])
