@@ -126,6 +126,19 @@ __host__ __device__ __forceinline__ int park_boundary(int n, int L, int P)
     return lo < n ? lo : n;
 }
 
+// Tile of block b when a launch has one block per tile (n of each).  Workgroups are dealt to
+// the 8 XCDs round-robin (block b runs on XCD b % 8), each XCD with its own L2; this order
+// gives XCD x the contiguous tiles [x*q + min(x, r), ...) (q, r = n / 8, n % 8) in dispatch
+// order, so that neighbouring tiles, which share the cache lines at their borders wherever
+// a row is not 128-byte aligned, meet in the same L2.  A bijection of [0, n).
+constexpr int kXcd = 8;
+__device__ __forceinline__ long long xcd_tile(long long b, long long n)
+{
+    const long long q = n / kXcd, r = n % kXcd;
+    const long long x = b % kXcd;
+    return x * q + min(x, r) + b / kXcd;
+}
+
 // Domain of global tile t: the largest i with prefix[i] <= t (prefix has ndom+1 entries).
 __device__ __forceinline__ int find_domain(const long long *prefix, int ndom, long long t)
 {
@@ -203,7 +216,10 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
     const long long ntiles = prefix[ndom];
     int tab_dom = -1, tab_s = -1;
 
-    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    // one block per tile: XCD-aware tile order (xcd_tile); a strided grid keeps its stride
+    const bool one_per_tile = (long long)gridDim.x == ntiles;
+    for (long long tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+        const long long tile = one_per_tile ? xcd_tile(tb, ntiles) : tb;
         const int di = __builtin_amdgcn_readfirstlane(find_domain(prefix, ndom, tile));  // uniform
         const DomainDesc<T> &D = descs[di];
         const int nx = D.nx, ny = D.ny, nz = D.nz;

@@ -106,3 +106,25 @@ def test_autotune_keeps_state_and_results():
     for i, c in enumerate(cases):
         assert np.array_equal(b.download(i), orc.run(c)), f"member {i}"
     b.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_phases_compose_a_step(precision):
+    """bfvmb_batch_launch_phase: sources/sinks, then the x, y and z sweeps, one phase at a
+    time, give the bits of bfvmb_batch_run's steps (FP64: the oracle's)."""
+    cases = _cases(3)
+    print(f"[gpu] batch phases {precision}", flush=True)
+    a = _batch(cases, precision)
+    a.run(3)
+    b = _batch(cases, precision)
+    for _ in range(3):
+        for phase in (0, 1, 2, 3):
+            b.launch_phase(phase, 1)
+    b.sync()
+    for i, c in enumerate(cases):
+        got = b.download(i)
+        assert np.array_equal(got, a.download(i)), f"member {i} {SHAPES[i]}"
+        if precision == "fp64":
+            assert np.array_equal(got, orc.run(c))
+    a.close()
+    b.close()
