@@ -477,7 +477,8 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
 //  * FP32 (3, 0): two wavefronts per SIMD and no register chunks, three chunks in flight (the
 //    kernel needs ~230 registers per lane without register chunks, so two wavefronts per SIMD
 //    leave no room for them: 8 register chunks spilled 392 bytes per lane; a (2, 16) variant,
-//    two chunks in flight beside 16 register chunks, measured no faster than (1, 17)).
+//    two chunks in flight beside 16 register chunks, measured no faster than (1, 17), with
+//    its tables by scalar loads or in LDS).
 // Each FP32 variant reads its Thomas tables (rden forward, cc backward) either with scalar
 // loads or, staged once per tile, from LDS (tl): scalar loads return out of order, so every
 // wait for an LDS access also waits for them, and the FP32 sweep, one wavefront per SIMD,
