@@ -622,14 +622,17 @@ int group_autotune(Group *g, int reps, int *chosen)
         const int *rings = axis == 0 ? x_rings : yz_rings;
         const int nring = axis == 0 ? (g->elem == 4 ? 4 : 1) : 2;
         const int bpcs[5] = {2, 3, 4, 6, 8};
-        // x also tries a grid of one block per wavefront slot (up to four per CU) striding
-        // over the tiles: the short lines of a batch run a few percent faster that way
+        // x tries one block per tile (a grid request above the tile count) and a grid of one
+        // block per wavefront slot (up to four per CU) striding over the tiles; y/z take the
+        // default, one block per tile
         const int ngrid = axis == 0 ? 2 : 1;
         for (int b = 0; b < 5 && !rc; ++b)
             for (int r = 0; r < nring && !rc; ++r)
                 for (int gi = 0; gi < ngrid && !rc; ++gi) {
                     g->launch[axis] = LaunchShape();
-                    g->launch[axis].grid = gi == 0 ? 0 : (long long)std::max(g->cu_count, 1) * std::min(bpcs[b], 4);
+                    if (axis == 0)
+                        g->launch[axis].grid =
+                            gi == 0 ? (1LL << 40) : (long long)std::max(g->cu_count, 1) * std::min(bpcs[b], 4);
                     g->launch[axis].lds_blocks_per_cu = bpcs[b];
                     g->launch[axis].ring = axis == 0 && g->elem == 8 ? 0 : rings[r];
                     if ((rc = launch_sweep(g, axis)))  // warm-up

@@ -484,10 +484,13 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
 // wait for an LDS access also waits for them, and the FP32 sweep, one wavefront per SIMD,
 // has no other wavefront to cover that; the LDS copy costs 16 bytes per column of line store.
 // The launch shape's ring value picks the FP32 variant: 1 = (1, 17), 2 = (1, 17) tl,
-// 3 = (3, 0), 4 = (3, 0) tl; ring 0 takes 1 at up to four wavefronts per CU and 3 above.
-// Measured: at 400^3 x 4, 1 and 2 run equally fast (0.652 / 0.656 ms: the tables cost a
-// chunk of line store, and a 400-column line then recomputes one chunk); on the batch of 64
-// uneven domains (lines of 32-256) 2 takes 2.41 ms against 2.65 ms for 1.
+// 3 = (3, 0), 4 = (3, 0) tl; ring 0 takes 2 at up to four wavefronts per CU and 3 above.
+// Measured at 400^3 x 4 (ms): 1 and 2 run equally fast with one block per tile (0.649 /
+// 0.654: the tables cost a chunk of line store, and a 400-column line then recomputes one
+// chunk), but with a grid of one block per SIMD striding over the tiles 2 stages its tables
+// once per substrate and takes 0.620 (1: 0.639), so that grid is the default for the
+// register-chunk variants.  On the batch of 64 uneven domains (lines of 32-256) 2 takes
+// 2.41 against 2.65 for 1.
 struct XVariant {
     int pc, regc;
     bool tl;
@@ -505,7 +508,7 @@ XVariant x_variant(int bpc, int ring)
         return {3, 0, false};
     if (ring == 4)
         return {3, 0, true};
-    return bpc <= 4 ? XVariant{1, 17, false} : XVariant{3, 0, false};
+    return bpc <= 4 ? XVariant{1, 17, true} : XVariant{3, 0, false};
 }
 
 // Wavefronts per SIMD an x-sweep variant is compiled for: two without register chunks (256
@@ -875,8 +878,8 @@ Shape shape_of(const Group *g, int axis)
     sh.grid = u.grid;
     // defaults measured on MI355X at 400^3 x 4 (with the head recomputation): four y/z sweep
     // wavefronts share a CU's LDS, six FP64 x sweep wavefronts (two per SIMD on some SIMDs),
-    // four FP32 x sweep wavefronts (one per SIMD, holding register chunks), and y/z keep 32
-    // rows per lane in flight
+    // four FP32 x sweep wavefronts (one per SIMD, holding register chunks, tables in LDS, a
+    // strided grid), and y/z keep 32 rows per lane in flight
     sh.bpc = u.lds_blocks_per_cu > 0 ? u.lds_blocks_per_cu : (axis == 0 && g->elem == 8 ? 6 : 4);
     sh.ring = u.ring > 0 ? u.ring : (axis == 0 ? 0 : 32);  // x: the variant's chunks in flight
     return sh;
@@ -985,7 +988,13 @@ int launch_sweep_t(Group *g, int axis)
     const long long ntiles = g->prefix[axis + 1][ndom];
     if (ntiles == 0)
         return BFVMB_OK;
-    long long grid = sh.grid > 0 ? std::min<long long>(sh.grid, ntiles) : ntiles;
+    // default grid: one block per tile, except for the FP32 x variants with register chunks
+    // (one wavefront per SIMD): one block per SIMD striding over the tiles
+    long long grid = ntiles;
+    if (sh.grid > 0)
+        grid = std::min<long long>(sh.grid, ntiles);
+    else if (axis == 0 && xv.regc > 0)
+        grid = std::min<long long>(ntiles, (long long)std::max(g->cu_count, 1) * std::min(sh.bpc, 4));
     grid = std::min<long long>(grid, 1LL << 30);
     const DomainDesc<T> *d = static_cast<const DomainDesc<T> *>(g->d_desc);
     if (axis == 0)
