@@ -586,8 +586,8 @@ int group_set_launch(Group *g, int sweep, long long grid, int bpc, int ring)
 }
 
 // Autotuner: for each sweep, time the candidate launch shapes (LDS wavefronts per CU; ring
-// depth for y/z; the variant and a strided grid for the x sweep) over `reps` launches with
-// HIP events and keep the fastest.  The fields are
+// depth for y/z; the variant for the FP32 x sweep; one block per tile or a strided grid)
+// over `reps` launches with HIP events and keep the fastest.  The fields are
 // saved before and restored after, so tuning does not advance the simulation.
 int group_autotune(Group *g, int reps, int *chosen)
 {
@@ -622,17 +622,15 @@ int group_autotune(Group *g, int reps, int *chosen)
         const int *rings = axis == 0 ? x_rings : yz_rings;
         const int nring = axis == 0 ? (g->elem == 4 ? 4 : 1) : 2;
         const int bpcs[5] = {2, 3, 4, 6, 8};
-        // x tries one block per tile (a grid request above the tile count) and a grid of one
-        // block per wavefront slot (up to four per CU) striding over the tiles; y/z take the
-        // default, one block per tile
-        const int ngrid = axis == 0 ? 2 : 1;
+        // each sweep tries one block per tile (a grid request above the tile count) and a grid
+        // of one block per wavefront slot (up to four per CU) striding over the tiles
+        const int ngrid = 2;
         for (int b = 0; b < 5 && !rc; ++b)
             for (int r = 0; r < nring && !rc; ++r)
                 for (int gi = 0; gi < ngrid && !rc; ++gi) {
                     g->launch[axis] = LaunchShape();
-                    if (axis == 0)
-                        g->launch[axis].grid =
-                            gi == 0 ? (1LL << 40) : (long long)std::max(g->cu_count, 1) * std::min(bpcs[b], 4);
+                    g->launch[axis].grid =
+                        gi == 0 ? (1LL << 40) : (long long)std::max(g->cu_count, 1) * std::min(bpcs[b], 4);
                     g->launch[axis].lds_blocks_per_cu = bpcs[b];
                     g->launch[axis].ring = axis == 0 && g->elem == 8 ? 0 : rings[r];
                     if ((rc = launch_sweep(g, axis)))  // warm-up

@@ -126,18 +126,12 @@ __host__ __device__ __forceinline__ int park_boundary(int n, int L, int P)
     return lo < n ? lo : n;
 }
 
-// Tile of block b when a launch has one block per tile (n of each).  Workgroups are dealt to
-// the 8 XCDs round-robin (block b runs on XCD b % 8), each XCD with its own L2; this order
-// gives XCD x the contiguous tiles [x*q + min(x, r), ...) (q, r = n / 8, n % 8) in dispatch
-// order, so that neighbouring tiles, which share the cache lines at their borders wherever
-// a row is not 128-byte aligned, meet in the same L2.  A bijection of [0, n).
+// XCD-aware tile order (y/z sweeps).  Workgroups are dealt to the 8 XCDs round-robin (block b
+// runs on XCD b % 8), each XCD with its own L2.  XCD x takes the contiguous tiles
+// [x*q + min(x, r), ...) (q + 1 of them for x < r; q, r = n / 8, n % 8) and its blocks
+// stride over them in dispatch order, so that neighbouring tiles, which share the cache
+// lines at their borders wherever a row is not 128-byte aligned, meet in the same L2.
 constexpr int kXcd = 8;
-__device__ __forceinline__ long long xcd_tile(long long b, long long n)
-{
-    const long long q = n / kXcd, r = n % kXcd;
-    const long long x = b % kXcd;
-    return x * q + min(x, r) + b / kXcd;
-}
 
 // Domain of global tile t: the largest i with prefix[i] <= t (prefix has ndom+1 entries).
 __device__ __forceinline__ int find_domain(const long long *prefix, int ndom, long long t)
@@ -216,10 +210,16 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
     const long long ntiles = prefix[ndom];
     int tab_dom = -1, tab_s = -1;
 
-    // one block per tile: XCD-aware tile order (xcd_tile); a strided grid keeps its stride
-    const bool one_per_tile = (long long)gridDim.x == ntiles;
-    for (long long tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
-        const long long tile = one_per_tile ? xcd_tile(tb, ntiles) : tb;
+    // XCD-aware tile order (one tile per block with one block per tile); a grid that would
+    // leave an XCD with tiles but no blocks strides over all tiles in order instead
+    const long long G = gridDim.x;
+    const bool by_xcd = G >= kXcd || G == ntiles;
+    const long long xq = ntiles / kXcd, xr = ntiles % kXcd, bx = blockIdx.x % kXcd;
+    const long long t_first = by_xcd ? bx * xq + min(bx, xr) : 0;                 // XCD's first tile
+    const long long t_count = by_xcd ? xq + (bx < xr ? 1 : 0) : ntiles;          // its tile count
+    const long long t_step = by_xcd ? G / kXcd + (bx < G % kXcd ? 1 : 0) : G;    // its blocks
+    for (long long k = by_xcd ? blockIdx.x / kXcd : blockIdx.x; k < t_count; k += t_step) {
+        const long long tile = t_first + k;
         const int di = __builtin_amdgcn_readfirstlane(find_domain(prefix, ndom, tile));  // uniform
         const DomainDesc<T> &D = descs[di];
         const int nx = D.nx, ny = D.ny, nz = D.nz;

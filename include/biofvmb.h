@@ -163,9 +163,9 @@ int bfvmb_launch_phase(bfvmb_ctx *ctx, int phase, int count);
 int bfvmb_set_launch(bfvmb_ctx *ctx, int sweep, int grid_blocks, int lds_blocks_per_cu, int ring_depth);
 
 /* Autotuner: times the candidate launch shapes of each sweep (LDS wavefronts per CU 2, 3,
- * 4, 6, 8; ring depth 32/64 for y/z; the FP32 x variants and, for x, a grid of one block
- * per wavefront slot striding over the tiles) over `reps` launches with HIP events and
- * keeps the fastest.
+ * 4, 6, 8; ring depth 32/64 for y/z; the FP32 x variants; one block per tile or a grid of
+ * one block per wavefront slot striding over the tiles) over `reps` launches with HIP
+ * events and keeps the fastest.
  * The field is saved and restored, so tuning does not advance the simulation.  `chosen`
  * (may be NULL) receives 3 x {wavefronts per CU, ring, microseconds per launch}. */
 int bfvmb_autotune(bfvmb_ctx *ctx, int reps, int *chosen);

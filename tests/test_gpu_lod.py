@@ -123,7 +123,7 @@ def test_set_agents_with_temps_and_inactive():
 
 
 @pytest.mark.parametrize("name", ["chunks_130x70x33x2", "long_x_700x5x6x2", "long_z_5x6x700x2"])
-@pytest.mark.parametrize("launch", [(7, 1, 16), (0, 2, 32), (3, 8, 64), (0, 4, 16)])
+@pytest.mark.parametrize("launch", [(7, 1, 16), (0, 2, 32), (3, 8, 64), (0, 4, 16), (64, 4, 32), (9, 6, 32)])
 def test_launch_shapes_do_not_change_results(name, launch):
     """Strided grids, ring depths and smaller LDS line stores (more of each line recomputed
     from checkpoints) give the same bits as the default launch: FP64 the oracle's, FP32 the
