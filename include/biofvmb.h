@@ -154,7 +154,7 @@ int bfvmb_launch_phase(bfvmb_ctx *ctx, int phase, int count);
 /* Launch shape of sweep `sweep` (0 = x, 1 = y, 2 = z, -1 = all three); 0 keeps the
  * library default: grid size (a grid smaller than the number of line tiles strides over
  * them; the default is one block per tile, except for the FP32 x-sweep variants with
- * register chunks: one block per SIMD), how many sweep
+ * register chunks and the FP32 z sweep: one block per SIMD), how many sweep
  * wavefronts share one CU's LDS line store, and rows in flight per lane (y/z: 16/32/64) or
  * the x-sweep kernel variant (FP32: 1 = one column chunk in flight and 17 register chunks,
  * 3 = three chunks in flight and none, each with its Thomas tables read by scalar loads;
