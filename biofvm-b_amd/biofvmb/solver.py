@@ -273,7 +273,8 @@ class LODSolver:
         return g0, g1, g2
 
     def autotune(self, reps: int = 3) -> np.ndarray:
-        """Pick each sweep's launch shape by timing; returns [[wavefronts/CU, ring, us]] x 3."""
+        """Pick each sweep's launch shape by timing; returns [[wavefronts/CU, ring, us]] x 3
+        (for the x sweep, "ring" is the kernel variant, see bfvmb_set_launch)."""
         out = np.zeros(9, dtype=np.int32)
         self._check(self._lib.bfvmb_autotune(self._h, reps, L.iptr(out)), "autotune")
         return out.reshape(3, 3)
@@ -383,6 +384,7 @@ class Batch:
         self._check(self._lib.bfvmb_batch_set_launch(self._h, sweep, grid, lds_blocks_per_cu, ring), "set_launch")
 
     def autotune(self, reps=3):
+        """bfvmb_batch_autotune: [[wavefronts/CU, ring or x variant, us]] x 3."""
         out = np.zeros(9, dtype=np.int32)
         self._check(self._lib.bfvmb_batch_autotune(self._h, reps, L.iptr(out)), "autotune")
         return out.reshape(3, 3)
