@@ -32,6 +32,7 @@ SIGNATURES = {
     "bfvmb_set_stream": (_c.c_int, [_vp, _vp]),
     "bfvmb_set_coefficients": (_c.c_int, [_vp, _dp, _dp, _c.c_double]),
     "bfvmb_set_dirichlet": (_c.c_int, [_vp, _dp, _bp]),
+    "bfvmb_set_dirichlet_nodes": (_c.c_int, [_vp, _c.c_int, _ip, _dp]),
     "bfvmb_upload_density": (_c.c_int, [_vp, _dp]),
     "bfvmb_download_density": (_c.c_int, [_vp, _dp]),
     "bfvmb_set_agents": (_c.c_int, [_vp, _c.c_int, _ip, _dp, _dp, _bp]),
@@ -64,6 +65,7 @@ SIGNATURES = {
     "bfvmb_batch_set_stream": (_c.c_int, [_vp, _vp]),
     "bfvmb_batch_set_coefficients": (_c.c_int, [_vp, _c.c_int, _dp, _dp, _c.c_double]),
     "bfvmb_batch_set_dirichlet": (_c.c_int, [_vp, _c.c_int, _dp, _bp]),
+    "bfvmb_batch_set_dirichlet_nodes": (_c.c_int, [_vp, _c.c_int, _c.c_int, _ip, _dp]),
     "bfvmb_batch_upload_density": (_c.c_int, [_vp, _c.c_int, _dp]),
     "bfvmb_batch_download_density": (_c.c_int, [_vp, _c.c_int, _dp]),
     "bfvmb_batch_set_agents": (_c.c_int, [_vp, _c.c_int, _c.c_int, _ip, _dp, _dp, _bp]),

@@ -98,9 +98,18 @@ class LODSolver:
         self._check(self._lib.bfvmb_set_coefficients(self._h, L.dptr(D), L.dptr(lam), float(dt)), "set_coefficients")
 
     def set_dirichlet(self, value, active) -> None:
-        v = np.ascontiguousarray(np.broadcast_to(value, (self.nsub,)), dtype=np.float64)
+        """One value per substrate on every face voxel (value None: activations only)."""
+        v = None if value is None else np.ascontiguousarray(np.broadcast_to(value, (self.nsub,)), dtype=np.float64)
         a = np.ascontiguousarray(np.broadcast_to(active, (self.nsub,)), dtype=np.uint8)
-        self._check(self._lib.bfvmb_set_dirichlet(self._h, L.dptr(v), L.bptr(a)), "set_dirichlet")
+        self._check(self._lib.bfvmb_set_dirichlet(self._h, None if v is None else L.dptr(v), L.bptr(a)),
+                    "set_dirichlet")
+
+    def set_dirichlet_nodes(self, voxel, values) -> None:
+        """Per-voxel Dirichlet values (add_dirichlet_node): voxel[n], values[n][nsub]."""
+        v = np.ascontiguousarray(voxel, dtype=np.int32).reshape(-1)
+        val = np.ascontiguousarray(values, dtype=np.float64).reshape(v.size, self.nsub)
+        self._check(self._lib.bfvmb_set_dirichlet_nodes(self._h, v.size, L.iptr(v), L.dptr(val)),
+                    "set_dirichlet_nodes")
 
     def _device_f64(self, t, what):
         import torch
@@ -354,6 +363,13 @@ class Batch:
         v = np.ascontiguousarray(np.broadcast_to(value, (ns,)), dtype=np.float64)
         a = np.ascontiguousarray(np.broadcast_to(active, (ns,)), dtype=np.uint8)
         self._check(self._lib.bfvmb_batch_set_dirichlet(self._h, i, L.dptr(v), L.bptr(a)), "set_dirichlet")
+
+    def set_dirichlet_nodes(self, i, voxel, values):
+        ns = self.shapes[i][3]
+        v = np.ascontiguousarray(voxel, dtype=np.int32).reshape(-1)
+        val = np.ascontiguousarray(values, dtype=np.float64).reshape(v.size, ns)
+        self._check(self._lib.bfvmb_batch_set_dirichlet_nodes(self._h, i, v.size, L.iptr(v), L.dptr(val)),
+                    "set_dirichlet_nodes")
 
     def upload(self, i, rho):
         r = np.ascontiguousarray(rho, dtype=np.float64)

@@ -130,13 +130,16 @@ __global__ __launch_bounds__(256) void k_agent_build(const unsigned *__restrict_
     }
     if (key[k] == dead)
         return;
+    // sid is the exclusive scan of head: a segment's first agent holds the segment's index, the
+    // others that index + 1
+    const int seg = sid[k] + head[k] - 1;
     if (head[k]) {
-        seg_start[sid[k]] = k;
-        seg_off[sid[k]] = (long long)key[k];
+        seg_start[seg] = k;
+        seg_off[seg] = (long long)key[k];
     }
     if (k + 1 == n0 || key[k + 1] == dead) {  // the last live agent closes the list
-        seg_start[sid[k] + 1] = k + 1;
-        counts[0] = sid[k] + 1;
+        seg_start[seg + 1] = k + 1;
+        counts[0] = seg + 1;
         counts[1] = k + 1;
     }
 }

@@ -39,6 +39,8 @@ struct DomainDesc {
     AxisTables tab[3];           // x, y, z tables
     const T *dval;               // [nsub]  Dirichlet value
     const uint8_t *dact;         // [nsub]  Dirichlet activation
+    const T *dface;              // per-voxel Dirichlet values of the six faces (face_x/y/z), or
+                                 // nullptr: dval for every voxel of a substrate
     const int *seg_start;        // agents: [nseg+1] offsets into the voxel-sorted list
     const long long *seg_off;    // agents: [nseg] voxel offset z*nx*ny + y*nx + x
     const double *t1, *t2;       // agents: [nsub][nact] temp1 / temp2, sorted order
@@ -48,6 +50,27 @@ struct DomainDesc {
     int nseg, nact;
     int nx, ny, nz, nsub;
 };
+
+// Per-voxel Dirichlet values of a domain's six faces, in one array of the field's precision:
+// the x faces as planes [2][nsub][nz][ny], then the y faces [2][nsub][nz][nx], then the z
+// faces [2][nsub][ny][nx] (e = 0 the low face, 1 the high one).  A voxel on an edge or corner
+// appears in several planes, with the same value.
+__host__ __device__ __forceinline__ size_t face_x(int ny, int nz, int ns, int e, int s, int y, int z)
+{
+    return (((size_t)e * ns + s) * nz + z) * ny + y;
+}
+__host__ __device__ __forceinline__ size_t face_y(int nx, int ny, int nz, int ns, int e, int s, int x, int z)
+{
+    return 2 * (size_t)ns * nz * ny + (((size_t)e * ns + s) * nz + z) * nx + x;
+}
+__host__ __device__ __forceinline__ size_t face_z(int nx, int ny, int nz, int ns, int e, int s, int x, int y)
+{
+    return 2 * (size_t)ns * nz * ((size_t)ny + nx) + (((size_t)e * ns + s) * ny + y) * nx + x;
+}
+__host__ __device__ __forceinline__ size_t face_total(int nx, int ny, int nz, int ns)
+{
+    return 2 * (size_t)ns * ((size_t)nz * ny + (size_t)nz * nx + (size_t)ny * nx);
+}
 
 // Launch shape of one sweep as set through bfvmb_set_launch (0 = library default).
 struct LaunchShape {
@@ -70,6 +93,8 @@ struct Domain {
     std::vector<uint8_t> dact;
     void *d_dval = nullptr;
     uint8_t *d_dact = nullptr;
+    std::vector<double> dface;     // per-voxel face values (face_x/y/z layout); empty: dval
+    void *d_dface = nullptr;       // their device copy in the field's precision
     // agents (device CSR)
     int nseg = 0, nact = 0;
     int *seg_start = nullptr;
@@ -143,6 +168,7 @@ int group_init(Group *g, int ndom, const int *nx, const int *ny, const int *nz, 
 void group_free(Group *g);
 int domain_set_coefficients(Group *g, int i, const double *D, const double *decay, double dt);
 int domain_set_dirichlet(Group *g, int i, const double *value, const uint8_t *active);
+int domain_set_dirichlet_nodes(Group *g, int i, int n, const int *voxel, const double *values);
 int domain_upload(Group *g, int i, const double *rho);
 int domain_download(Group *g, int i, double *rho);
 int domain_set_agents(Group *g, int i, int n, const int *voxel, const double *temp1, const double *temp2,

@@ -66,6 +66,11 @@ int bfvmb_set_dirichlet(bfvmb_ctx *ctx, const double *value, const uint8_t *acti
     return ctx ? domain_set_dirichlet(ctx, 0, value, active) : BFVMB_ERR_ARG;
 }
 
+int bfvmb_set_dirichlet_nodes(bfvmb_ctx *ctx, int n, const int *voxel, const double *values)
+{
+    return ctx ? domain_set_dirichlet_nodes(ctx, 0, n, voxel, values) : BFVMB_ERR_ARG;
+}
+
 int bfvmb_upload_density(bfvmb_ctx *ctx, const double *rho) { return ctx ? domain_upload(ctx, 0, rho) : BFVMB_ERR_ARG; }
 
 int bfvmb_download_density(bfvmb_ctx *ctx, double *rho) { return ctx ? domain_download(ctx, 0, rho) : BFVMB_ERR_ARG; }
@@ -249,6 +254,11 @@ int bfvmb_batch_set_coefficients(bfvmb_batch *b, int i, const double *diffusion,
 int bfvmb_batch_set_dirichlet(bfvmb_batch *b, int i, const double *value, const uint8_t *active)
 {
     return b ? domain_set_dirichlet(b, i, value, active) : BFVMB_ERR_ARG;
+}
+
+int bfvmb_batch_set_dirichlet_nodes(bfvmb_batch *b, int i, int n, const int *voxel, const double *values)
+{
+    return b ? domain_set_dirichlet_nodes(b, i, n, voxel, values) : BFVMB_ERR_ARG;
 }
 
 int bfvmb_batch_upload_density(bfvmb_batch *b, int i, const double *rho)

@@ -93,6 +93,7 @@ std::vector<DomainDesc<T>> build_descs(Group *g)
         }
         e.dval = static_cast<const T *>(d.d_dval);
         e.dact = d.d_dact;
+        e.dface = d.dface.empty() ? nullptr : static_cast<const T *>(d.d_dface);
         e.seg_start = d.seg_start;
         e.seg_off = d.seg_off;
         e.t1 = d.t1;
@@ -240,6 +241,7 @@ void group_free(Group *g)
         (void)hipFree(d.coef);
         (void)hipFree(d.d_dval);
         (void)hipFree(d.d_dact);
+        (void)hipFree(d.d_dface);
     }
     adjoint_free(g);
     g->doms.clear();
@@ -309,14 +311,94 @@ int domain_set_coefficients(Group *g, int i, const double *D, const double *deca
 
 int domain_set_dirichlet(Group *g, int i, const double *value, const uint8_t *active)
 {
-    if (bad_index(g, i) || !value || !active)
+    if (bad_index(g, i) || !active)
         return set_error(g, BFVMB_ERR_ARG, "set_dirichlet: bad argument");
     Domain &d = g->doms[i];
-    d.dval.assign(value, value + d.nsub);
+    if (value) {
+        // one value per substrate on every face voxel: set_substrate_dirichlet_value writes it
+        // to every voxel (BioFVM_microenvironment.cpp:345-351), replacing per-voxel values
+        int rc = hip_check(g, hipStreamSynchronize(g->stream), "dirichlet sync");
+        if (rc)
+            return rc;
+        d.dval.assign(value, value + d.nsub);
+        if (!d.dface.empty()) {
+            std::vector<double>().swap(d.dface);
+            (void)hipFree(d.d_dface);
+            d.d_dface = nullptr;
+            g->desc_dirty = true;
+        }
+    }
     for (int s = 0; s < d.nsub; ++s)
         d.dact[s] = active[s] ? 1 : 0;
     g->epoch++;
     return upload_dirichlet(g, d);
+}
+
+// add_dirichlet_node / update_dirichlet_node (BioFVM_microenvironment.cpp:226-290): values of
+// listed voxels.  apply_dirichlet_conditions (:381-484) reads dirichlet_value_vectors on the six
+// faces only, so the face planes hold everything the LOD step can see; interior voxels are
+// accepted and change nothing.
+int domain_set_dirichlet_nodes(Group *g, int i, int n, const int *voxel, const double *values)
+{
+    if (bad_index(g, i) || n < 0 || (n > 0 && (!voxel || !values)))
+        return set_error(g, BFVMB_ERR_ARG, "set_dirichlet_nodes: bad argument");
+    Domain &d = g->doms[i];
+    const int nx = d.nx, ny = d.ny, nz = d.nz, ns = d.nsub;
+    for (int k = 0; k < n; ++k)
+        if (voxel[k] < 0 || (size_t)voxel[k] >= d.nvox)
+            return set_error(g, BFVMB_ERR_ARG, "set_dirichlet_nodes: voxel index out of range");
+    std::vector<double> f = d.dface;
+    if (f.empty()) {  // start from the per-substrate values
+        f.resize(face_total(nx, ny, nz, ns));
+        for (int e = 0; e < 2; ++e)
+            for (int s = 0; s < ns; ++s) {
+                const double v = d.dval[s];
+                std::fill_n(f.begin() + face_x(ny, nz, ns, e, s, 0, 0), (size_t)nz * ny, v);
+                std::fill_n(f.begin() + face_y(nx, ny, nz, ns, e, s, 0, 0), (size_t)nz * nx, v);
+                std::fill_n(f.begin() + face_z(nx, ny, nz, ns, e, s, 0, 0), (size_t)ny * nx, v);
+            }
+    }
+    const long long nyz = (long long)ny * nz;
+    for (int k = 0; k < n; ++k) {
+        const long long v = voxel[k];  // reference numbering (x*ny + y)*nz + z
+        const int x = (int)(v / nyz), y = (int)((v - x * nyz) / nz), z = (int)(v - x * nyz - (long long)y * nz);
+        const double *val = values + (size_t)k * ns;
+        for (int s = 0; s < ns; ++s) {
+            for (int e = 0; e < 2; ++e) {
+                if (x == (e ? nx - 1 : 0))
+                    f[face_x(ny, nz, ns, e, s, y, z)] = val[s];
+                if (y == (e ? ny - 1 : 0))
+                    f[face_y(nx, ny, nz, ns, e, s, x, z)] = val[s];
+                if (z == (e ? nz - 1 : 0))
+                    f[face_z(nx, ny, nz, ns, e, s, x, y)] = val[s];
+            }
+        }
+    }
+    int rc = hip_check(g, hipStreamSynchronize(g->stream), "dirichlet sync");
+    if (rc)
+        return rc;
+    if (!d.d_dface && (rc = hip_check(g, hipMalloc(&d.d_dface, f.size() * g->elem), "dirichlet faces")))
+        return rc;
+    if (g->elem == 8) {
+        rc = hip_check(g, hipMemcpy(d.d_dface, f.data(), f.size() * 8, hipMemcpyHostToDevice), "dirichlet faces");
+    } else {
+        std::vector<float> h(f.begin(), f.end());
+        rc = hip_check(g, hipMemcpy(d.d_dface, h.data(), h.size() * 4, hipMemcpyHostToDevice), "dirichlet faces");
+    }
+    if (rc) {  // back to the per-substrate values rather than half-written planes
+        (void)hipFree(d.d_dface);
+        d.d_dface = nullptr;
+        std::vector<double>().swap(d.dface);
+        g->desc_dirty = true;
+        g->epoch++;
+        return rc;
+    }
+    const bool was_on = !d.dface.empty();
+    d.dface.swap(f);
+    if (!was_on)
+        g->desc_dirty = true;
+    g->epoch++;
+    return BFVMB_OK;
 }
 
 static int ensure_stage(Group *g, size_t bytes)
@@ -658,18 +740,27 @@ int group_autotune(Group *g, int reps, int *chosen)
         (void)hipEventDestroy(e0);
     if (e1)
         (void)hipEventDestroy(e1);
+    // restore whatever was saved, also after a failure part-way through the timing (the sweeps
+    // that ran have advanced the field); a field that could not be restored is a new state
+    bool restored = true;
     for (size_t i = 0; i < g->doms.size(); ++i) {
         if (!saved[i])
             continue;
         const Domain &d = g->doms[i];
-        if (!rc)
-            rc = hip_check(g, hipMemcpyAsync(d.field[start], saved[i], d.nvox * d.nsub * g->elem,
-                                             hipMemcpyDeviceToDevice, g->stream), "autotune restore");
+        const int rr = hip_check(g, hipMemcpyAsync(d.field[start], saved[i], d.nvox * d.nsub * g->elem,
+                                                   hipMemcpyDeviceToDevice, g->stream), "autotune restore");
+        if (rr) {
+            restored = false;
+            if (!rc)
+                rc = rr;
+        }
     }
     int rc2 = hip_check(g, hipStreamSynchronize(g->stream), "autotune sync");
     for (void *p : saved)
         (void)hipFree(p);
     g->cur = start;
+    if (!restored || rc2)
+        g->epoch++;
     clear_graphs(g);
     return rc ? rc : rc2;
 }

@@ -26,7 +26,8 @@
 //    r = RN(1/d) from the host.  With q faithful after the first correction, Markstein's
 //    theorem makes the second result the correctly rounded a/d, i.e. bit-identical to the
 //    reference's division, with a 6-deep instead of a ~10-deep dependent chain.  Numerators
-//    below 2^-960 (outside the theorem's no-underflow range) take a true division.
+//    below 2^-960 (outside the theorem's no-underflow range), infinities and NaNs take a true
+//    division.
 //  * FP32 storage path: the recurrence runs in FP64 on values converted on load (with the
 //    FP64 reciprocal), results are rounded to FP32 once when stored.
 //  * Dirichlet is fused: the x sweep applies it on load (the reference's first call) and
@@ -85,7 +86,8 @@ __device__ __forceinline__ double get_ck(gmem<T> p, size_t off, size_t off2)
 }
 
 // a/d correctly rounded from r = RN(1/d): q = a*r, two corrections (Markstein).  Valid for
-// |a| >= 2^-960 (no underflow in the corrections); callers flag smaller numerators.
+// finite |a| >= 2^-960 (no underflow in the corrections; an infinite a would give NaN where
+// the division gives +-inf); callers flag the other numerators.
 __device__ __forceinline__ double div_corr(double a, double d, double r)
 {
     double q = a * r;
@@ -94,21 +96,29 @@ __device__ __forceinline__ double div_corr(double a, double d, double r)
     e = fma(-q, d, a);
     return fma(e, r, q);
 }
-// Smallest numerator the corrected reciprocal handles, 2^-960, as the upper 32 bits of its
-// pattern (the lower ones are zero): |a| < 2^-960 exactly when hi32(|a|) < kTinyHi.
-constexpr unsigned kTinyHi = (1023u - 960u) << 20;
+// Numerator range check in one integer operation per division: with h = the upper 32 bits
+// of a's pattern, key(a) = (h << 1) - (kTinyHi << 1) (mod 2^32; the shift drops the sign) is
+// below kKeyBad exactly when 2^-960 <= |a| < inf, i.e. when div_corr is exact.  Tiny
+// numerators wrap round to the top of the range, infinities and NaNs (h >= 0x7ff00000) land
+// at or above kKeyBad.  A line keeps the largest key it met.
+constexpr unsigned kTinyHi = (1023u - 960u) << 20;  // upper half of 2^-960
+constexpr unsigned kKeyBad = (0x7ff00000u << 1) - (kTinyHi << 1);
+__device__ __forceinline__ unsigned num_key(double a)
+{
+    return ((unsigned)((unsigned long long)__double_as_longlong(a) >> 32) << 1) - (kTinyHi << 1);
+}
 
 // One forward step in double: y = fma(c1, prev, v) / den.  With prev = -0.0 the same
 // expression gives the first element's v / den[0] exactly, signed zeros included.
-// FP64 storage: corrected reciprocal; `amin` tracks the smallest upper half of |numerator|
-// (two integer operations) so that a numerator outside its range is detected once per line;
-// FP32 storage: the FP64 reciprocal.
+// FP64 storage: corrected reciprocal; `akey` keeps the largest num_key of the line's
+// numerators (two integer operations) so that a numerator outside its range is detected once
+// per line; FP32 storage: the FP64 reciprocal.
 template <class T>
-__device__ __forceinline__ double fwd(double c1, double prev, double v, double den, double rden, unsigned &amin)
+__device__ __forceinline__ double fwd(double c1, double prev, double v, double den, double rden, unsigned &akey)
 {
     const double num = fma(c1, prev, v);
     if constexpr (sizeof(T) == 8) {
-        amin = min(amin, (unsigned)((unsigned long long)__double_as_longlong(num) >> 32) & 0x7fffffffu);
+        akey = max(akey, num_key(num));
         return div_corr(num, den, rden);
     } else {
         (void)den;
@@ -196,6 +206,37 @@ constexpr int reg_rows() { return sizeof(T) == 8 ? 160 : 320; }
 template <class T>
 constexpr int table_batch() { return sizeof(T) == 8 ? 8 : 16; }
 
+// Per-voxel Dirichlet values (descriptor dface) of a y/z line's Dirichlet voxels, stored over
+// what the sweep stored there; the lane itself wrote those elements, so program order makes
+// these stores the last.  The line (x, o) runs along y (AXIS 1, o = z) or z (AXIS 2, o = y):
+// its two ends lie on that axis' faces, and a line with x or o on a face lies wholly on it.
+template <class T, int AXIS>
+__device__ __attribute__((noinline)) void yz_face_values(const DomainDesc<T> &D, int s, gmem<T> dst, size_t stride,
+                                                         int n, int x, int o, bool face)
+{
+    const int nx = D.nx, ny = D.ny, nz = D.nz, ns = D.nsub;
+    const T *fv = D.dface;
+    if (AXIS == 1) {
+        dst[0] = fv[face_y(nx, ny, nz, ns, 0, s, x, o)];
+        dst[(size_t)(n - 1) * stride] = fv[face_y(nx, ny, nz, ns, 1, s, x, o)];
+    } else {
+        dst[0] = fv[face_z(nx, ny, nz, ns, 0, s, x, o)];
+        dst[(size_t)(n - 1) * stride] = fv[face_z(nx, ny, nz, ns, 1, s, x, o)];
+    }
+    if (!face)
+        return;
+    for (int k = 0; k < n; ++k) {
+        size_t i;
+        if (x == 0 || x == nx - 1)
+            i = AXIS == 1 ? face_x(ny, nz, ns, x == 0 ? 0 : 1, s, k, o) : face_x(ny, nz, ns, x == 0 ? 0 : 1, s, o, k);
+        else if (AXIS == 1)
+            i = face_z(nx, ny, nz, ns, o == 0 ? 0 : 1, s, x, k);
+        else
+            i = face_y(nx, ny, nz, ns, o == 0 ? 0 : 1, s, x, k);
+        dst[(size_t)k * stride] = fv[i];
+    }
+}
+
 template <class T, int AXIS, int P>
 __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict__ descs,
                                                  const long long *__restrict__ prefix, int ndom, int parity,
@@ -281,7 +322,7 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
             tab_s = s;
         }
         double y = -0.0;
-        unsigned amin = 0xffffffffu;
+        unsigned akey = 0u;
         T regs[RC * P];  // register rows [lo, lds_lo)
         // one full chunk of P rows from k0: prefetches row min(k + P, lim) (PF), stores the
         // results (STORE: 1 = line store at row k - base, 2 = registers rp[j])
@@ -305,7 +346,7 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
                     if constexpr (EXACT)
                         y = fma(c1, y, v) / t[jj].x;
                     else
-                        y = fwd<T>(c1, y, v, t[jj].x, t[jj].y, amin);
+                        y = fwd<T>(c1, y, v, t[jj].x, t[jj].y, akey);
                     if constexpr (STORE == 1)
                         ls[(size_t)(k - base) * kWave + lane] = (T)y;
                     else if constexpr (STORE == 2)
@@ -338,17 +379,17 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
                 const int k = k0 + j;
                 if (k < n) {
                     const double2 t = tab2[k];
-                    y = fwd<T>(c1, y, (double)buf[j], t.x, t.y, amin);
+                    y = fwd<T>(c1, y, (double)buf[j], t.x, t.y, akey);
                     ls[(size_t)(k - lds_lo) * kWave + lane] = (T)y;
                 }
             }
         }
         bool exact = false;
         if constexpr (sizeof(T) == 8) {
-            // a numerator below 2^-960 somewhere in the tile: redo the forward pass with
-            // true divisions (same results everywhere else); the head recomputation below
+            // a numerator below 2^-960 or not finite somewhere in the tile: redo the forward
+            // pass with true divisions (same results everywhere else); the head recomputation below
             // then divides the same way
-            if (__builtin_expect(__any(amin < kTinyHi), 0)) {
+            if (__builtin_expect(__any(akey >= kKeyBad), 0)) {
                 exact = true;
                 y = -0.0;
                 for (int k = 0; k < lo; ++k) {
@@ -458,6 +499,8 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
             back(b - 1, a);
             b = a;
         }
+        if (act && D.dface != nullptr)  // per-voxel face values (uniform branch)
+            yz_face_values<T, AXIS>(D, s, dst, stride, n, x, o, face);
     }
 }
 
@@ -519,7 +562,9 @@ constexpr int x_waves_per_simd()
     return REGC == 0 ? 2 : 1;
 }
 
-template <class T, int PC, int REGC, bool TL>
+// VF: the kernel also applies per-voxel Dirichlet values (descriptor dface) where a domain has
+// them; a separate instantiation, so that the per-substrate case keeps its registers.
+template <class T, int PC, int REGC, bool TL, bool VF>
 __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(const DomainDesc<T> *__restrict__ descs,
                                                 const long long *__restrict__ prefix, int ndom, int parity,
                                                 int budget)
@@ -572,6 +617,9 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
         const bool act = __builtin_amdgcn_readfirstlane((int)D.dact[s]) != 0;
         const T dv = ((const __attribute__((address_space(4))) T *)D.dval)[s];
         const bool whole = act && (yy == 0 || yy == ny - 1 || zz == 0 || zz == nz - 1);
+        // per-voxel Dirichlet values (descriptor dface): read where they are used, inside
+        // uniform branches, so that the per-substrate case keeps no registers for them
+        const bool vf = VF && act && D.dface != nullptr;
         const gmem<const T> src = as_global<const T>(D.field[parity] + (size_t)s * nvox + (size_t)r0 * nx);
         const gmem<T> dst = as_global<T>(D.field[parity ^ 1] + (size_t)s * nvox + (size_t)r0 * nx);
         const double c1 = as_ctab(D.c1)[s];
@@ -607,6 +655,15 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
 #pragma unroll
             for (int i = 0; i < NI; ++i)
                 stg[(RPI * i + lrow) * (W + 1) + lcol] = edge ? dv : r[i];
+        };
+        // this lane's row of the staging tile gets its x-face values (chunk 0: column 0, last
+        // chunk: column nx - 1); stored after put_chunk / the substitution wrote that element,
+        // by the same wavefront, so the LDS keeps this value
+        auto put_xfaces = [&](int c) {
+            if (c == 0)
+                stg[lane * (W + 1)] = D.dface[face_x(ny, nz, D.nsub, 0, s, yy, zz)];
+            if (c == nch - 1)
+                stg[lane * (W + 1) + (nx - 1 - c * W)] = D.dface[face_x(ny, nz, D.nsub, 1, s, yy, zz)];
         };
         auto store_chunk = [&](int c) {  // staging tile -> dst, coalesced
             if (full_rows && c * W + W <= nx) {
@@ -650,11 +707,13 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
         double y = -0.0;
         // chunk c: stage the ring slot, refill it with chunk `next`, then W recurrence steps;
         // results (STORE) 1: to the line store at chunk c - base, 2: to registers rp[j]
-        auto fwd_chunk = [&](int c, T *slot, int next, int base, auto store_tag, auto exact_tag, unsigned &amin,
+        auto fwd_chunk = [&](int c, T *slot, int next, int base, auto store_tag, auto exact_tag, unsigned &akey,
                              T *rp) __attribute__((always_inline)) {
             constexpr int STORE = decltype(store_tag)::value;
             constexpr bool EXACT = decltype(exact_tag)::value;
             put_chunk(slot, c);
+            if (vf && (c == 0 || c == nch - 1))
+                put_xfaces(c);
             if (next >= 0)  // next < 0: nothing left to prefetch into this slot
                 load_chunk(src, next, slot);
             __syncthreads();
@@ -677,7 +736,7 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
                 if constexpr (EXACT)
                     y = fma(c1, y, v[j]) / dd[j];
                 else
-                    y = fwd<T>(c1, y, v[j], dd[j], rd[j], amin);
+                    y = fwd<T>(c1, y, v[j], dd[j], rd[j], akey);
                 if constexpr (STORE == 1)
                     tgt[j] = (T)y;
                 else if constexpr (STORE == 2)
@@ -701,7 +760,7 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
         const std::false_type inexact;
         const std::true_type exact_div;
         auto forward = [&](auto exact_tag) {
-            unsigned amin = 0xffffffffu;
+            unsigned akey = 0u;
 #pragma unroll
             for (int p = 0; p < PC; ++p)
                 load_chunk(src, min(p, nch - 1), ring[p]);
@@ -729,9 +788,9 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
                         const int c = c0 + p;
                         if (c < nch) {
                             if (c < cl)
-                                fwd_chunk(c, ring[p], min(c + PC, nch - 1), 0, no_store, exact_tag, amin, nullptr);
+                                fwd_chunk(c, ring[p], min(c + PC, nch - 1), 0, no_store, exact_tag, akey, nullptr);
                             else
-                                fwd_chunk(c, ring[p], min(c + PC, nch - 1), cl, store, exact_tag, amin, nullptr);
+                                fwd_chunk(c, ring[p], min(c + PC, nch - 1), cl, store, exact_tag, akey, nullptr);
                             if (c + 1 == next_ck && next_ck < cr) {  // checkpoint at a segment end
                                 put_ck<T>(dst, ck_off(next_ck), ck_off2(next_ck), y);
                                 next_ck += Sc;
@@ -744,7 +803,7 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
 #pragma unroll
                     for (int p = 0; p < PC; ++p) {
                         const int c = c0 + p;
-                        fwd_chunk(c, ring[p], min(c + PC, nch - 1), 0, no_store, exact_tag, amin, nullptr);
+                        fwd_chunk(c, ring[p], min(c + PC, nch - 1), 0, no_store, exact_tag, akey, nullptr);
                         if (c + 1 == next_ck && next_ck < cr) {  // checkpoint at a segment end
                             put_ck<T>(dst, ck_off(next_ck), ck_off2(next_ck), y);
                             next_ck += Sc;
@@ -754,28 +813,28 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
 #pragma unroll
                 for (int q = 0; q < REGC; ++q)  // register chunks: chunk cr + q in ring slot q % PC
                     if (cr + q < cl)
-                        fwd_chunk(cr + q, ring[q % PC], min(cr + q + PC, nch - 1), 0, reg_store, exact_tag, amin,
+                        fwd_chunk(cr + q, ring[q % PC], min(cr + q + PC, nch - 1), 0, reg_store, exact_tag, akey,
                                   regs + q * W);
                 for (int c0 = cl; c0 < nch; c0 += PC) {  // line store (cl is a multiple of PC)
 #pragma unroll
                     for (int p = 0; p < PC; ++p) {
                         const int c = c0 + p;
                         if (c < nch)
-                            fwd_chunk(c, ring[p], min(c + PC, nch - 1), cl, store, exact_tag, amin, nullptr);
+                            fwd_chunk(c, ring[p], min(c + PC, nch - 1), cl, store, exact_tag, akey, nullptr);
                     }
                 }
             }
-            return amin;
+            return akey;
         };
         bool exact = false;
-        const unsigned amin = forward(inexact);
+        const unsigned akey = forward(inexact);
         if constexpr (sizeof(T) == 8) {
-            if (__builtin_expect(__any(amin < kTinyHi), 0)) {
+            if (__builtin_expect(__any(akey >= kKeyBad), 0)) {
                 exact = true;
                 forward(exact_div);
             }
         } else {
-            (void)amin;
+            (void)akey;
         }
 
         // ---- backward: chunk c's forward results sit in the line store at c - base, or in
@@ -809,6 +868,17 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
                     if (j < w)
                         bstep(j);
             }
+            if (vf) {
+                if (c == 0 || c == nch - 1)
+                    put_xfaces(c);
+                if (whole) {  // a row on a y or z face: its values from that face's plane
+                    const T *row = zz == 0 || zz == nz - 1
+                                       ? D.dface + face_z(nx, ny, nz, D.nsub, zz == 0 ? 0 : 1, s, 0, yy)
+                                       : D.dface + face_y(nx, ny, nz, D.nsub, yy == 0 ? 0 : 1, s, 0, zz);
+                    for (int j = 0; j < w; ++j)
+                        stg[lane * (W + 1) + j] = row[c * W + j];
+                }
+            }
             __syncthreads();
             store_chunk(c);
             __syncthreads();
@@ -840,7 +910,7 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
         for (int cb = cr; cb > 0;) {
             const int ca = seg_begin(cb);
             y = yck;
-            unsigned unused = 0xffffffffu;
+            unsigned unused = 0u;
             for (int c0 = ca; c0 < cb; c0 += PC) {
 #pragma unroll
                 for (int p = 0; p < PC; ++p) {
@@ -897,24 +967,49 @@ void launch_yz(const Group *g, const DomainDesc<T> *d, const long long *pre, int
         hipLaunchKernelGGL((k_sweep_yz<T, AXIS, 64>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
 }
 
+template <class T, bool VF>
+void launch_x_vf(const Group *g, const DomainDesc<T> *d, const long long *pre, int ndom, unsigned grid, size_t lds,
+                 XVariant xv)
+{
+    if constexpr (sizeof(T) == 8) {
+        (void)xv;
+        hipLaunchKernelGGL((k_sweep_x<T, 2, 0, false, VF>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+    } else if (xv.pc == 1) {
+        if (xv.tl)
+            hipLaunchKernelGGL((k_sweep_x<T, 1, 17, true, VF>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+        else
+            hipLaunchKernelGGL((k_sweep_x<T, 1, 17, false, VF>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+    } else {
+        if (xv.tl)
+            hipLaunchKernelGGL((k_sweep_x<T, 3, 0, true, VF>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+        else
+            hipLaunchKernelGGL((k_sweep_x<T, 3, 0, false, VF>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+    }
+}
+
 template <class T>
 void launch_x(const Group *g, const DomainDesc<T> *d, const long long *pre, int ndom, unsigned grid, size_t lds,
               XVariant xv)
 {
-    if constexpr (sizeof(T) == 8) {
-        (void)xv;
-        hipLaunchKernelGGL((k_sweep_x<T, 2, 0, false>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
-    } else if (xv.pc == 1) {
-        if (xv.tl)
-            hipLaunchKernelGGL((k_sweep_x<T, 1, 17, true>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
-        else
-            hipLaunchKernelGGL((k_sweep_x<T, 1, 17, false>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
-    } else {
-        if (xv.tl)
-            hipLaunchKernelGGL((k_sweep_x<T, 3, 0, true>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
-        else
-            hipLaunchKernelGGL((k_sweep_x<T, 3, 0, false>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
-    }
+    bool vf = false;  // some domain has per-voxel Dirichlet values
+    for (const Domain &dm : g->doms)
+        vf = vf || !dm.dface.empty();
+    if (vf)
+        launch_x_vf<T, true>(g, d, pre, ndom, grid, lds, xv);
+    else
+        launch_x_vf<T, false>(g, d, pre, ndom, grid, lds, xv);
+}
+
+template <bool VF>
+hipError_t x_attributes(int bytes)
+{
+    const void *k[5] = {(const void *)k_sweep_x<double, 2, 0, false, VF>, (const void *)k_sweep_x<float, 1, 17, false, VF>,
+                        (const void *)k_sweep_x<float, 1, 17, true, VF>, (const void *)k_sweep_x<float, 3, 0, false, VF>,
+                        (const void *)k_sweep_x<float, 3, 0, true, VF>};
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 5 && e == hipSuccess; ++i)
+        e = hipFuncSetAttribute(k[i], hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    return e;
 }
 
 template <class T>
@@ -1044,15 +1139,9 @@ int sweep_lds_attributes(size_t lds_bytes)
     if (e == hipSuccess)
         e = yz_attributes<float, 2>(bytes);
     if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_sweep_x<double, 2, 0, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        e = x_attributes<false>(bytes);
     if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_sweep_x<float, 1, 17, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_sweep_x<float, 1, 17, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_sweep_x<float, 3, 0, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_sweep_x<float, 3, 0, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        e = x_attributes<true>(bytes);
     return e == hipSuccess ? 0 : (int)e;
 }
 

@@ -10,7 +10,7 @@
 // followed by simulate_diffusion_decay(dt).
 //
 // usage: bfvmb_driver <case.bin> <out.bin> [fp64|fp32] [--fused] [--time] [--mat out.mat]
-//                     [--bulk rates.bin] [--sample samples.bin]
+//                     [--bulk rates.bin] [--sample samples.bin] [--dnodes nodes.bin]
 // --sample: at every snapshot, before the field is written, the densities at the agents'
 // voxels through Microenvironment::nearest_density_vectors (read from the device field),
 // [nagents][nsub] doubles per snapshot.
@@ -62,7 +62,12 @@ int main(int argc, char **argv)
     const char *mat_path = nullptr;   // --mat PATH: write_to_matlab after the last step
     const char *bulk_path = nullptr;  // --bulk PATH: supply, target, uptake [nvox][nsub] doubles
     const char *sample_path = nullptr;  // --sample PATH: densities at the agents' voxels per snapshot
+    const char *dnodes_path = nullptr;  // --dnodes PATH: n, voxel[n], values[n][nsub] Dirichlet nodes
     for (int i = 3; i < argc; ++i) {
+        if (!std::strcmp(argv[i], "--dnodes") && i + 1 < argc) {
+            dnodes_path = argv[++i];
+            continue;
+        }
         if (!std::strcmp(argv[i], "--mat") && i + 1 < argc)
             mat_path = argv[++i];
         else if (!std::strcmp(argv[i], "--bulk") && i + 1 < argc)
@@ -125,6 +130,29 @@ int main(int argc, char **argv)
                 continue;  // keep resize_space's defaults (100.0, active)
             M.set_substrate_dirichlet_value(d, dval[d]);
             M.set_substrate_dirichlet_activation(d, dact[d] != 0);
+        }
+        if (dnodes_path) {  // the four node setters in turn, as the reference driver does
+            FILE *q = std::fopen(dnodes_path, "rb");
+            if (!q) {
+                std::perror(dnodes_path);
+                return 2;
+            }
+            const int n = one<int>(q);
+            std::vector<int> vox(n);
+            std::vector<double> val((size_t)n * nsub);
+            get(q, vox.data(), vox.size());
+            get(q, val.data(), val.size());
+            std::fclose(q);
+            for (int k = 0; k < n; ++k) {
+                std::vector<double> v(val.begin() + (size_t)k * nsub, val.begin() + (size_t)(k + 1) * nsub);
+                const int x = vox[k] / (ny * nz), y = (vox[k] / nz) % ny, z = vox[k] % nz;
+                switch (k % 4) {
+                case 0: M.add_dirichlet_node(vox[k], v); break;
+                case 1: M.add_dirichlet_node(x, y, z, v); break;
+                case 2: M.update_dirichlet_node(x, y, z, v); break;
+                default: M.update_dirichlet_node(vox[k], v); break;
+                }
+            }
         }
         std::copy(init.begin(), init.end(), M.density_data());
 

@@ -1,17 +1,15 @@
 // bfvmb_reference_plugin.cpp -- the reference application's hot-path entry points, backed by
 // the MI355X library (see bfvmb_reference_plugin.hpp).  Compiled against the application's
-// BioFVM-B headers; it reads only members those headers make public, plus the agents'
-// protected source/sink constants through a pointer to member of a derived class (legal
-// C++, no header change needed).
-//
-// Dirichlet values and activations are private to Microenvironment and have no getter, so
-// they are read back through the public apply_dirichlet_conditions (BioFVM_microenvironment
-// .cpp:381-484): the plugin marks one face voxel with NaN, lets the reference overwrite it,
-// and restores it.  The first call checks every face voxel this way (the library takes one
-// value per substrate, which is what every public setter produces).
+// BioFVM-B headers without changing them.  It reads the members those headers make public,
+// the agents' protected source/sink constants through a pointer to member of a derived
+// class, and three private members -- Microenvironment::dirichlet_value_vectors and
+// dirichlet_activation_vector (BioFVM_microenvironment.h:137-139), which no getter exposes, and
+// Basic_Agent::volume_is_changed (BioFVM_basic_agent.h:68) -- through pointers to member that
+// an explicit template instantiation hands out (the usual access checks do not apply to names
+// used in an explicit instantiation, [temp.explicit]).
 #include <cmath>
 #include <cstdio>
-#include <limits>
+#include <cstring>
 #include <map>
 #include <vector>
 
@@ -29,6 +27,26 @@ struct AgentAccess : Basic_Agent {
     static bool Basic_Agent::*active() { return &AgentAccess::is_active; }
 };
 
+template <class Tag, typename Tag::type P>
+struct Expose {
+    friend typename Tag::type member(Tag) { return P; }
+};
+struct DirichletValues {
+    typedef std::vector<double> Microenvironment::*type;
+    friend type member(DirichletValues);
+};
+struct DirichletActivation {
+    typedef std::vector<bool> Microenvironment::*type;
+    friend type member(DirichletActivation);
+};
+struct VolumeChanged {
+    typedef bool Basic_Agent::*type;
+    friend type member(VolumeChanged);
+};
+template struct Expose<DirichletValues, &Microenvironment::dirichlet_value_vectors>;
+template struct Expose<DirichletActivation, &Microenvironment::dirichlet_activation_vector>;
+template struct Expose<VolumeChanged, &Basic_Agent::volume_is_changed>;
+
 struct PluginState {
     bfvmb_ctx *ctx = nullptr;
     int nx = 0, ny = 0, nz = 0, nsub = 0;
@@ -36,11 +54,17 @@ struct PluginState {
     bool resident = false;
     bool device_current = false;  // the device holds the newest field
     bool host_current = true;     // p_density_vectors holds the newest field
-    std::vector<double> dval;
-    std::vector<uint8_t> dact;
-    std::vector<double> agent_volume;  // volumes seen at the last upload
-    const std::vector<Basic_Agent *> *agents_key = nullptr;
-    std::vector<double> bulk[3];       // bulk supply / target / uptake at the last upload
+    bool dirichlet_pushed = false;
+    bool rescan_dirichlet = true;  // read dirichlet_value_vectors again at the next call
+    std::vector<uint8_t> dact;     // activations last pushed
+    std::vector<int> face_vox;     // the face voxels, reference numbering
+    std::vector<double> face_val;  // dirichlet_value_vectors there ([face voxel][substrate]), last pushed
+    // agents as last uploaded: voxel, active flag, temp1/temp2 [agent][substrate]
+    std::vector<int> agent_voxel;
+    std::vector<uint8_t> agent_active;
+    std::vector<double> agent_t1, agent_t2;
+    bool agents_uploaded = false;
+    std::vector<double> bulk[3];  // bulk supply / target / uptake at the last upload
 };
 
 std::map<Microenvironment *, PluginState> g_state;
@@ -51,48 +75,67 @@ bool fail(const char *what, bfvmb_ctx *ctx)
     return false;
 }
 
-// Dirichlet value/activation per substrate as apply_dirichlet_conditions would apply them.
-bool probe_dirichlet(Microenvironment &M, PluginState &S, bool full)
+bool same_value(double a, double b) { return a == b || (std::isnan(a) && std::isnan(b)); }
+
+// The Dirichlet data apply_dirichlet_conditions (BioFVM_microenvironment.cpp:381-484) applies:
+// the activation per substrate, and dirichlet_value_vectors on the six faces (values of
+// interior voxels never reach the field).  A substrate whose face voxels all hold one value
+// goes to the library as that value; otherwise every face voxel goes as a Dirichlet node.
+// Values are read when scan_values is set; data equal to the last push are not pushed again.
+bool push_dirichlet(Microenvironment &M, PluginState &S, bool scan_values)
 {
     const int ns = S.nsub;
-    std::vector<double> &rho = *M.p_density_vectors;
-    const double nan = std::numeric_limits<double>::quiet_NaN();
-    std::vector<size_t> vox;
-    if (full) {
-        for (int i = 0; i < S.nx; ++i)
-            for (int j = 0; j < S.ny; ++j)
-                for (int k = 0; k < S.nz; ++k)
-                    if (i == 0 || j == 0 || k == 0 || i == S.nx - 1 || j == S.ny - 1 || k == S.nz - 1)
-                        vox.push_back(((size_t)i * S.ny + j) * S.nz + k);
-    } else {
-        vox.push_back(0);
+    const size_t nvox = (size_t)S.nx * S.ny * S.nz;
+    const std::vector<bool> &act = M.*member(DirichletActivation());
+    const std::vector<double> &val = M.*member(DirichletValues());
+    if ((int)act.size() < ns || val.size() < nvox * ns) {
+        std::printf("Error: bfvmb plugin: Dirichlet vectors are not sized for the mesh and substrates\n");
+        return false;
     }
-    std::vector<double> saved(vox.size() * ns);
-    for (size_t v = 0; v < vox.size(); ++v)
-        for (int d = 0; d < ns; ++d) {
-            saved[v * ns + d] = rho[vox[v] * ns + d];
-            rho[vox[v] * ns + d] = nan;
-        }
-    M.apply_dirichlet_conditions(0, 1);
-    S.dval.assign(ns, 0.0);
-    S.dact.assign(ns, 0);
-    bool ok = true;
+    std::vector<uint8_t> a(ns);
+    for (int d = 0; d < ns; ++d)
+        a[d] = act[d] ? 1 : 0;
+    bool values_changed = false;
+    if (scan_values || !S.dirichlet_pushed) {
+        if (S.face_vox.empty())
+            for (int i = 0; i < S.nx; ++i)
+                for (int j = 0; j < S.ny; ++j)
+                    for (int k = 0; k < S.nz; ++k)
+                        if (i == 0 || j == 0 || k == 0 || i == S.nx - 1 || j == S.ny - 1 || k == S.nz - 1)
+                            S.face_vox.push_back((i * S.ny + j) * S.nz + k);
+        const size_t nf = S.face_vox.size();
+        std::vector<double> fv(nf * ns);
+        for (size_t v = 0; v < nf; ++v)
+            std::memcpy(&fv[v * ns], &val[(size_t)S.face_vox[v] * ns], ns * sizeof(double));
+        values_changed = !S.dirichlet_pushed || fv.size() != S.face_val.size() ||
+                         std::memcmp(fv.data(), S.face_val.data(), fv.size() * sizeof(double)) != 0;
+        if (values_changed)
+            S.face_val.swap(fv);
+    }
+    if (!values_changed) {
+        if (a == S.dact)
+            return true;
+        if (bfvmb_set_dirichlet(S.ctx, nullptr, a.data()) != BFVMB_OK)
+            return fail("set_dirichlet", S.ctx);
+        S.dact = a;
+        return true;
+    }
+    const size_t nf = S.face_vox.size();
+    std::vector<double> dval(ns);
+    bool uniform = true;
     for (int d = 0; d < ns; ++d) {
-        const double v0 = rho[vox[0] * ns + d];
-        S.dact[d] = std::isnan(v0) ? 0 : 1;
-        S.dval[d] = S.dact[d] ? v0 : 0.0;
-        for (size_t v = 1; v < vox.size() && ok; ++v) {
-            const double w = rho[vox[v] * ns + d];
-            if (std::isnan(w) != std::isnan(v0) || (!std::isnan(w) && w != v0))
-                ok = false;
-        }
+        dval[d] = S.face_val[d];
+        for (size_t v = 1; v < nf && uniform; ++v)
+            uniform = same_value(S.face_val[v * ns + d], dval[d]);
     }
-    for (size_t v = 0; v < vox.size(); ++v)
-        for (int d = 0; d < ns; ++d)
-            rho[vox[v] * ns + d] = saved[v * ns + d];
-    if (!ok)
-        std::printf("Error: bfvmb plugin: Dirichlet values differ between face voxels of one substrate\n");
-    return ok;
+    S.dirichlet_pushed = false;
+    if (bfvmb_set_dirichlet(S.ctx, dval.data(), a.data()) != BFVMB_OK)
+        return fail("set_dirichlet", S.ctx);
+    if (!uniform && bfvmb_set_dirichlet_nodes(S.ctx, (int)nf, S.face_vox.data(), S.face_val.data()) != BFVMB_OK)
+        return fail("set_dirichlet_nodes", S.ctx);
+    S.dact = a;
+    S.dirichlet_pushed = true;
+    return true;
 }
 
 PluginState *state_for(Microenvironment &M)
@@ -164,17 +207,13 @@ void bfvmb_hip_LOD_3D(Microenvironment &M, double dt, int mpi_Size, int mpi_Rank
             fail("set_coefficients", S->ctx);
             return;
         }
-        if (!probe_dirichlet(M, *S, true))
-            return;
         S->coef_done = true;
-    } else if (!S->resident) {
-        if (!probe_dirichlet(M, *S, false))  // values may change between calls
-            return;
     }
-    if (bfvmb_set_dirichlet(S->ctx, S->dval.data(), S->dact.data()) != BFVMB_OK) {
-        fail("set_dirichlet", S->ctx);
+    // activations every call; values every call while the field is synced each call, and
+    // after bfvmb_plugin_invalidate when it is resident
+    if (!push_dirichlet(M, *S, !S->resident || S->rescan_dirichlet))
         return;
-    }
+    S->rescan_dirichlet = false;
     if (!to_device(M, *S))
         return;
     if (bfvmb_simulate_diffusion_decay(S->ctx, 1) != BFVMB_OK) {
@@ -191,36 +230,51 @@ void bfvmb_hip_cell_sources_and_sinks(Microenvironment &M, std::vector<Basic_Age
     PluginState *S = state_for(M);
     if (!S)
         return;
-    // An agent whose volume changed since the last upload gets fresh constants first, as
-    // simulate_secretion_and_uptake does for volume_is_changed (BioFVM_basic_agent.cpp:285-289).
-    const size_t n = agents.size();
-    const bool same_list = S->agents_key == &agents && S->agent_volume.size() == n;
+    // The agents' voxels, flags and constants, gathered in parallel.  An active agent whose
+    // volume changed gets fresh constants first, as simulate_secretion_and_uptake does
+    // (BioFVM_basic_agent.cpp:285-289).  The list goes to the device only when it differs from
+    // the last upload.
+    const long n = (long)agents.size();
     const int ns = S->nsub;
     std::vector<int> voxel(n);
-    std::vector<double> t1(n * ns), t2(n * ns);
+    std::vector<double> t1((size_t)n * ns), t2((size_t)n * ns);
     std::vector<uint8_t> act(n);
-    for (size_t i = 0; i < n; ++i) {
+    const bool comparable = S->agents_uploaded && S->agent_voxel.size() == (size_t)n;
+    int changed = comparable ? 0 : 1;
+#pragma omp parallel for schedule(static) reduction(| : changed)
+    for (long i = 0; i < n; ++i) {
         Basic_Agent *a = agents[i];
-        if (same_list && a->get_total_volume() != S->agent_volume[i])
+        const bool live = a->*AgentAccess::active();
+        if (live && a->*member(VolumeChanged()))
             a->set_internal_uptake_constants(dt);
         const std::vector<double> &a1 = a->*AgentAccess::t1();
         const std::vector<double> &a2 = a->*AgentAccess::t2();
-        voxel[i] = a->get_current_voxel_index();
-        act[i] = (a->*AgentAccess::active()) && voxel[i] >= 0 ? 1 : 0;
-        if (voxel[i] < 0)
-            voxel[i] = 0;
+        int v = a->get_current_voxel_index();
+        act[i] = live && v >= 0 ? 1 : 0;
+        voxel[i] = v < 0 ? 0 : v;
         for (int d = 0; d < ns; ++d) {
-            t1[i * ns + d] = a1[d];
-            t2[i * ns + d] = a2[d];
+            t1[(size_t)i * ns + d] = a1[d];
+            t2[(size_t)i * ns + d] = a2[d];
+        }
+        if (comparable && !changed) {
+            int c = voxel[i] != S->agent_voxel[i] || act[i] != S->agent_active[i];
+            for (int d = 0; d < ns && !c; ++d)
+                c = std::memcmp(&t1[(size_t)i * ns + d], &S->agent_t1[(size_t)i * ns + d], sizeof(double)) != 0 ||
+                    std::memcmp(&t2[(size_t)i * ns + d], &S->agent_t2[(size_t)i * ns + d], sizeof(double)) != 0;
+            changed |= c;
         }
     }
-    S->agents_key = &agents;
-    S->agent_volume.resize(n);
-    for (size_t i = 0; i < n; ++i)
-        S->agent_volume[i] = agents[i]->get_total_volume();
-    if (bfvmb_set_agents(S->ctx, (int)n, voxel.data(), t1.data(), t2.data(), act.data()) != BFVMB_OK) {
-        fail("set_agents", S->ctx);
-        return;
+    if (changed) {
+        S->agents_uploaded = false;
+        if (bfvmb_set_agents(S->ctx, (int)n, voxel.data(), t1.data(), t2.data(), act.data()) != BFVMB_OK) {
+            fail("set_agents", S->ctx);
+            return;
+        }
+        S->agent_voxel.swap(voxel);
+        S->agent_active.swap(act);
+        S->agent_t1.swap(t1);
+        S->agent_t2.swap(t2);
+        S->agents_uploaded = true;
     }
     if (!to_device(M, *S))
         return;
@@ -289,8 +343,10 @@ void bfvmb_plugin_sync(Microenvironment &M)
 void bfvmb_plugin_invalidate(Microenvironment &M)
 {
     auto it = g_state.find(&M);
-    if (it != g_state.end())
+    if (it != g_state.end()) {
         it->second.device_current = false;
+        it->second.rescan_dirichlet = true;
+    }
 }
 
 void bfvmb_plugin_write_to_matlab(Microenvironment &M, const std::string &filename)

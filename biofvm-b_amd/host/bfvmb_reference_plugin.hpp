@@ -12,6 +12,11 @@
 // current (exactly the reference's contract).  bfvmb_plugin_set_resident(M, true) keeps the
 // field on the device between calls instead; bfvmb_plugin_sync(M) then brings the host
 // copy up to date and bfvmb_plugin_invalidate(M) uploads host-side edits on the next call.
+// Dirichlet data: activations are read at every call; the values (dirichlet_value_vectors on
+// the six faces, per-voxel values of add_dirichlet_node / update_dirichlet_node included) at
+// every call while the field is synced each call, and in resident mode at the first call and
+// after bfvmb_plugin_invalidate(M).  The agent list goes to the device only when a voxel,
+// active flag or source/sink constant differs from the last upload.
 #pragma once
 
 #include <string>

@@ -130,6 +130,7 @@ void Microenvironment::resize_densities(int n)
     decay_rates.assign(n, 0.0);
     dval_.assign(n, 1.0);
     dact_.assign(n, 1);
+    dnodes_.clear();
     host_.assign((size_t)number_of_voxels() * n, 0.0);
     if (ctx_) {
         bfvmb_destroy(ctx_);
@@ -166,6 +167,7 @@ void Microenvironment::resize_space_uniform(double x_start, double x_end, double
     z0_ = z_start;
     host_.assign((size_t)number_of_voxels() * number_of_densities(), 0.0);
     dval_.assign(number_of_densities(), 100.0);
+    dnodes_.clear();
     if (ctx_) {
         bfvmb_destroy(ctx_);
         ctx_ = nullptr;
@@ -192,6 +194,20 @@ void Microenvironment::set_substrate_dirichlet_value(int substrate, double value
         return;
     }
     dval_[substrate] = value;
+    for (auto &kv : dnodes_)  // the reference writes the value to every voxel
+        kv.second[substrate] = value;
+    dirichlet_dirty_ = true;
+}
+
+void Microenvironment::add_dirichlet_node(int voxel, const std::vector<double> &value)
+{
+    if (voxel < 0 || voxel >= number_of_voxels())
+        throw Error(BFVMB_ERR_ARG, "add_dirichlet_node: voxel index out of range");
+    auto it = dnodes_.find(voxel);
+    if (it == dnodes_.end())
+        it = dnodes_.emplace(voxel, dval_).first;
+    for (size_t d = 0; d < value.size() && d < it->second.size(); ++d)
+        it->second[d] = value[d];
     dirichlet_dirty_ = true;
 }
 
@@ -218,6 +234,19 @@ void Microenvironment::push_dirichlet()
     if (!dirichlet_dirty_)
         return;
     check(bfvmb_set_dirichlet(ctx_, dval_.data(), dact_.data()), "bfvmb_set_dirichlet");
+    if (!dnodes_.empty()) {
+        const size_t ns = dval_.size();
+        std::vector<int> voxel;
+        std::vector<double> values;
+        voxel.reserve(dnodes_.size());
+        values.reserve(dnodes_.size() * ns);
+        for (const auto &kv : dnodes_) {
+            voxel.push_back(kv.first);
+            values.insert(values.end(), kv.second.begin(), kv.second.end());
+        }
+        check(bfvmb_set_dirichlet_nodes(ctx_, (int)voxel.size(), voxel.data(), values.data()),
+              "bfvmb_set_dirichlet_nodes");
+    }
     dirichlet_dirty_ = false;
 }
 

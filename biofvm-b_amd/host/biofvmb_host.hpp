@@ -14,6 +14,7 @@
 // transfers explicit.
 #pragma once
 
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -104,6 +105,20 @@ public:
     void set_substrate_dirichlet_activation(int substrate, bool active);
     double dirichlet_value(int substrate) const { return dval_[substrate]; }
     bool dirichlet_activation(int substrate) const { return dact_[substrate] != 0; }
+    // per-voxel values (BioFVM_microenvironment.cpp:226-290): dirichlet_value_vectors at one
+    // voxel, its first value.size() substrates; apply_dirichlet_conditions applies them on
+    // the six faces only, so values of interior voxels never reach the field
+    void add_dirichlet_node(int voxel_index, const std::vector<double> &value);
+    void add_dirichlet_node(int x, int y, int z, const std::vector<double> &value)
+    {
+        add_dirichlet_node(voxel_index(x, y, z), value);
+    }
+    void update_dirichlet_node(int voxel_index, const std::vector<double> &value) { add_dirichlet_node(voxel_index, value); }
+    void update_dirichlet_node(int x, int y, int z, const std::vector<double> &value)
+    {
+        add_dirichlet_node(voxel_index(x, y, z), value);
+    }
+    bool is_dirichlet_node(int voxel_index) const { return dnodes_.count(voxel_index) != 0; }
 
     // ---- densities in the reference layout [x][y][z][substrate]
     double *density_vector(int n);
@@ -159,6 +174,7 @@ private:
     double dx_ = 1.0, x0_ = 0.0, y0_ = 0.0, z0_ = 0.0;
     std::vector<double> dval_;
     std::vector<uint8_t> dact_;
+    std::map<int, std::vector<double>> dnodes_;  // voxel -> dirichlet_value_vectors there
     std::vector<double> host_;  // p_density_vectors
     bool host_stale_ = false;   // device is newer than host_
     bool host_dirty_ = true;    // host_ may be newer than the device

@@ -60,8 +60,20 @@ int bfvmb_set_coefficients(bfvmb_ctx *ctx, const double *diffusion, const double
 
 /* Dirichlet value and activation per substrate: set_substrate_dirichlet_value /
  * set_substrate_dirichlet_activation (BioFVM_microenvironment.cpp:328-359).  Applied to all
- * six faces before and between the sweeps, as apply_dirichlet_conditions (:381-484). */
+ * six faces before and between the sweeps, as apply_dirichlet_conditions (:381-484).
+ * value[s] becomes the value of every voxel of substrate s, replacing values set per voxel
+ * with bfvmb_set_dirichlet_nodes (set_substrate_dirichlet_value writes every voxel); value ==
+ * NULL changes the activations only and keeps the values. */
 int bfvmb_set_dirichlet(bfvmb_ctx *ctx, const double *value, const uint8_t *active);
+
+/* Per-voxel Dirichlet values: Microenvironment::add_dirichlet_node / update_dirichlet_node
+ * (BioFVM_microenvironment.cpp:226-290, declared BioFVM_microenvironment.h:253-256) for n
+ * voxels (reference numbering) with values[k][substrate], i.e. dirichlet_value_vectors at those
+ * voxels.  apply_dirichlet_conditions (:381-484) applies dirichlet_value_vectors on the six
+ * faces only, for the substrates whose activation is on, so face voxels take these values in
+ * every later step; interior voxels are accepted and leave the step unchanged.  Voxels not
+ * listed keep their values (initially the per-substrate ones). */
+int bfvmb_set_dirichlet_nodes(bfvmb_ctx *ctx, int n, const int *voxel, const double *values);
 
 /* Host <-> device field transfer in the reference layout (p_density_vectors). */
 int bfvmb_upload_density(bfvmb_ctx *ctx, const double *rho_voxel_major);
@@ -245,6 +257,8 @@ int bfvmb_batch_set_stream(bfvmb_batch *b, void *stream);
 int bfvmb_batch_set_coefficients(bfvmb_batch *b, int i, const double *diffusion, const double *decay,
                                  double dt);
 int bfvmb_batch_set_dirichlet(bfvmb_batch *b, int i, const double *value, const uint8_t *active);
+/* bfvmb_set_dirichlet_nodes for member i. */
+int bfvmb_batch_set_dirichlet_nodes(bfvmb_batch *b, int i, int n, const int *voxel, const double *values);
 int bfvmb_batch_upload_density(bfvmb_batch *b, int i, const double *rho_voxel_major);
 int bfvmb_batch_download_density(bfvmb_batch *b, int i, double *rho_voxel_major);
 int bfvmb_batch_set_agents(bfvmb_batch *b, int i, int n, const int *voxel, const double *temp1,

@@ -82,33 +82,54 @@ ORC_API void orc_thomas_axis(int n, int nsub, const double *c1, const double *c2
 }
 
 /* apply_dirichlet_conditions, BioFVM_microenvironment.cpp:381-484: overwrite every voxel
- * on all six faces, for each substrate whose activation flag is set.  The value array is
- * per substrate: every public setter (set_substrate_dirichlet_value, :334-359, and the
- * 100.0 / 1.0 defaults of resize_space / resize_densities) fills it uniformly.          */
-ORC_API void orc_apply_dirichlet(int nx, int ny, int nz, int nsub, double *rho,
-                                 const double *value, const unsigned char *active)
+ * on all six faces (x faces, then y, then z), for each substrate whose activation flag is
+ * set, with dirichlet_value_vectors[voxel*nsub + d].  `dvox` is that per-voxel array
+ * ([nvox][nsub], as add_dirichlet_node / update_dirichlet_node write it, :226-290); NULL
+ * means every voxel of substrate d holds value[d] (set_substrate_dirichlet_value, :334-359,
+ * and the 100.0 / 1.0 defaults of resize_space / resize_densities).                      */
+static inline double orc_dval(const double *value, const double *dvox, size_t i, int d)
 {
+    return dvox ? dvox[i] : value[d];
+}
+
+ORC_API void orc_apply_dirichlet_voxels(int nx, int ny, int nz, int nsub, double *rho,
+                                        const double *value, const double *dvox,
+                                        const unsigned char *active)
+{
+    size_t i;
     for (int y = 0; y < ny; ++y)
         for (int z = 0; z < nz; ++z)
             for (int d = 0; d < nsub; ++d)
                 if (active[d]) {
-                    rho[orc_idx(0, y, z, d, ny, nz, nsub)] = value[d];
-                    rho[orc_idx(nx - 1, y, z, d, ny, nz, nsub)] = value[d];
+                    i = orc_idx(0, y, z, d, ny, nz, nsub);
+                    rho[i] = orc_dval(value, dvox, i, d);
+                    i = orc_idx(nx - 1, y, z, d, ny, nz, nsub);
+                    rho[i] = orc_dval(value, dvox, i, d);
                 }
     for (int x = 0; x < nx; ++x)
         for (int z = 0; z < nz; ++z)
             for (int d = 0; d < nsub; ++d)
                 if (active[d]) {
-                    rho[orc_idx(x, 0, z, d, ny, nz, nsub)] = value[d];
-                    rho[orc_idx(x, ny - 1, z, d, ny, nz, nsub)] = value[d];
+                    i = orc_idx(x, 0, z, d, ny, nz, nsub);
+                    rho[i] = orc_dval(value, dvox, i, d);
+                    i = orc_idx(x, ny - 1, z, d, ny, nz, nsub);
+                    rho[i] = orc_dval(value, dvox, i, d);
                 }
     for (int x = 0; x < nx; ++x)
         for (int y = 0; y < ny; ++y)
             for (int d = 0; d < nsub; ++d)
                 if (active[d]) {
-                    rho[orc_idx(x, y, 0, d, ny, nz, nsub)] = value[d];
-                    rho[orc_idx(x, y, nz - 1, d, ny, nz, nsub)] = value[d];
+                    i = orc_idx(x, y, 0, d, ny, nz, nsub);
+                    rho[i] = orc_dval(value, dvox, i, d);
+                    i = orc_idx(x, y, nz - 1, d, ny, nz, nsub);
+                    rho[i] = orc_dval(value, dvox, i, d);
                 }
+}
+
+ORC_API void orc_apply_dirichlet(int nx, int ny, int nz, int nsub, double *rho,
+                                 const double *value, const unsigned char *active)
+{
+    orc_apply_dirichlet_voxels(nx, ny, nz, nsub, rho, value, NULL, active);
 }
 
 /* One Thomas solve along a strided line: forward elimination
@@ -156,18 +177,26 @@ ORC_API void orc_sweep(int axis, int nx, int ny, int nz, int nsub, double *rho,
 
 /* diffusion_decay_solver__constant_coefficients_LOD_3D, one call after setup
  * (BioFVM_parallel.cpp:929-1392): Dirichlet, x, Dirichlet, y, Dirichlet, z, Dirichlet.  */
+static void orc_lod_step_dv(int nx, int ny, int nz, int nsub, double *rho, const double *c1,
+                            const double *denomx, const double *cx, const double *denomy,
+                            const double *cy, const double *denomz, const double *cz,
+                            const double *dvalue, const double *dvox, const unsigned char *dactive)
+{
+    orc_apply_dirichlet_voxels(nx, ny, nz, nsub, rho, dvalue, dvox, dactive);
+    orc_sweep(0, nx, ny, nz, nsub, rho, c1, denomx, cx);
+    orc_apply_dirichlet_voxels(nx, ny, nz, nsub, rho, dvalue, dvox, dactive);
+    orc_sweep(1, nx, ny, nz, nsub, rho, c1, denomy, cy);
+    orc_apply_dirichlet_voxels(nx, ny, nz, nsub, rho, dvalue, dvox, dactive);
+    orc_sweep(2, nx, ny, nz, nsub, rho, c1, denomz, cz);
+    orc_apply_dirichlet_voxels(nx, ny, nz, nsub, rho, dvalue, dvox, dactive);
+}
+
 ORC_API void orc_lod_step(int nx, int ny, int nz, int nsub, double *rho, const double *c1,
                           const double *denomx, const double *cx, const double *denomy,
                           const double *cy, const double *denomz, const double *cz,
                           const double *dvalue, const unsigned char *dactive)
 {
-    orc_apply_dirichlet(nx, ny, nz, nsub, rho, dvalue, dactive);
-    orc_sweep(0, nx, ny, nz, nsub, rho, c1, denomx, cx);
-    orc_apply_dirichlet(nx, ny, nz, nsub, rho, dvalue, dactive);
-    orc_sweep(1, nx, ny, nz, nsub, rho, c1, denomy, cy);
-    orc_apply_dirichlet(nx, ny, nz, nsub, rho, dvalue, dactive);
-    orc_sweep(2, nx, ny, nz, nsub, rho, c1, denomz, cz);
-    orc_apply_dirichlet(nx, ny, nz, nsub, rho, dvalue, dactive);
+    orc_lod_step_dv(nx, ny, nz, nsub, rho, c1, denomx, cx, denomy, cy, denomz, cz, dvalue, NULL, dactive);
 }
 
 /* Basic_Agent::set_internal_uptake_constants, BioFVM_basic_agent.cpp:155-176:
@@ -300,12 +329,14 @@ ORC_API int orc_voxel_index(const double *pos, const double *bbox, double dx, do
 
 /* A whole run as the BioFVM-B demos drive it (complete_diffusion_demo.cpp:489-493):
  * per step, cell sources/sinks (when with_agents) then one LOD step.  The Thomas tables
- * are built once from D, decay, dt and dx, as the solver's first call does. */
-ORC_API void orc_run(int nx, int ny, int nz, int nsub, double dx, double dt, const double *D,
-                     const double *decay, const double *dvalue, const unsigned char *dactive,
-                     int nagents, const int *voxel, const unsigned char *active,
-                     const double *temp1, const double *temp2, int nsteps, double *rho,
-                     double *work /* >= (3*(nx+ny+nz) + 4) * nsub doubles */)
+ * are built once from D, decay, dt and dx, as the solver's first call does.  dvox: the
+ * per-voxel Dirichlet values ([nvox][nsub]) or NULL (dvalue per substrate). */
+ORC_API void orc_run_dv(int nx, int ny, int nz, int nsub, double dx, double dt, const double *D,
+                        const double *decay, const double *dvalue, const double *dvox,
+                        const unsigned char *dactive, int nagents, const int *voxel,
+                        const unsigned char *active, const double *temp1, const double *temp2,
+                        int nsteps, double *rho,
+                        double *work /* >= (3*(nx+ny+nz) + 4) * nsub doubles */)
 {
     double *c1 = work, *c2 = c1 + nsub, *c3 = c2 + nsub, *c3a = c3 + nsub;
     double *denx = c3a + nsub, *cx = denx + (size_t)nx * nsub;
@@ -318,6 +349,16 @@ ORC_API void orc_run(int nx, int ny, int nz, int nsub, double dx, double dt, con
     for (int s = 0; s < nsteps; ++s) {
         if (nagents > 0)
             orc_sources_sinks(nagents, nsub, voxel, active, temp1, temp2, rho);
-        orc_lod_step(nx, ny, nz, nsub, rho, c1, denx, cx, deny, cy, denz, cz, dvalue, dactive);
+        orc_lod_step_dv(nx, ny, nz, nsub, rho, c1, denx, cx, deny, cy, denz, cz, dvalue, dvox, dactive);
     }
+}
+
+ORC_API void orc_run(int nx, int ny, int nz, int nsub, double dx, double dt, const double *D,
+                     const double *decay, const double *dvalue, const unsigned char *dactive,
+                     int nagents, const int *voxel, const unsigned char *active,
+                     const double *temp1, const double *temp2, int nsteps, double *rho,
+                     double *work)
+{
+    orc_run_dv(nx, ny, nz, nsub, dx, dt, D, decay, dvalue, NULL, dactive, nagents, voxel, active, temp1,
+               temp2, nsteps, rho, work);
 }

@@ -43,6 +43,7 @@ class Case:
     with_agents: bool = True
     x0: float = 0.0
     bulk: object = None  # (supply, target, uptake), each [nx, ny, nz, nsub]; bulk update per step
+    dnodes: object = None  # (voxel[n], values[n][nsub]): per-voxel Dirichlet values, in list order
 
     @property
     def nagents(self) -> int:
@@ -58,6 +59,14 @@ class Case:
         with open(path, "wb") as f:
             for a in self.bulk:
                 np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+
+    def write_dnodes(self, path: str) -> None:
+        """Per-voxel Dirichlet nodes as the drivers' --dnodes file: n, voxel[n], values[n][nsub]."""
+        voxel, values = self.dnodes
+        with open(path, "wb") as f:
+            np.array([len(voxel)], dtype=np.int32).tofile(f)
+            np.ascontiguousarray(voxel, dtype=np.int32).tofile(f)
+            np.ascontiguousarray(values, dtype=np.float64).tofile(f)
 
     def write(self, path: str) -> None:
         n = self.nagents
@@ -122,6 +131,10 @@ def run_reference(case: Case, workdir: str, solver: str = "lod", granularity: in
         bpath = os.path.join(workdir, "bulk.bin")
         case.write_bulk(bpath)
         args += ["--bulk", bpath]
+    if case.dnodes is not None:
+        dpath = os.path.join(workdir, "dnodes.bin")
+        case.write_dnodes(dpath)
+        args += ["--dnodes", dpath]
     proc = subprocess.run(args, env=env, capture_output=True, text=True, check=True)
     out = read_output(opath, case)
     out["stdout"] = proc.stdout

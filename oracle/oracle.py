@@ -41,6 +41,8 @@ def lib():
         L.orc_voxel_index.argtypes = [_dp, _dp, d, d, d, i, i, i]
         L.orc_voxel_index.restype = i
         L.orc_run.argtypes = [i, i, i, i, d, d, _dp, _dp, _dp, _bp, i, _ip, _bp, _dp, _dp, i, _dp, _dp]
+        L.orc_run_dv.argtypes = [i, i, i, i, d, d, _dp, _dp, _dp, _dp, _bp, i, _ip, _bp, _dp, _dp, i, _dp, _dp]
+        L.orc_apply_dirichlet_voxels.argtypes = [i, i, i, i, _dp, _dp, _dp, _bp]
         L.orc_bulk_sources_sinks.argtypes = [ctypes.c_size_t, i, d, _dp, _dp, _dp, _dp]
         L.orc_gradients.argtypes = [i, i, i, i, d, _dp, _dp]
         _lib = L
@@ -175,16 +177,34 @@ def run(case, nsteps=None, rho=None, with_agents=None, tables=None):
     k1, p1 = _d(t1 if na else np.zeros(1))
     k2, p2 = _d(t2 if na else np.zeros(1))
     work = np.empty((3 * (case.nx + case.ny + case.nz) + 4) * case.nsub)
+    dvox = dirichlet_voxels(case)
+    pvox = None if dvox is None else dvox.ctypes.data_as(_dp)
     if case.bulk is None:
-        lib().orc_run(case.nx, case.ny, case.nz, case.nsub, case.dx, case.dt, pD, pdc, pdv, pda, na, pv, pa,
-                      p1, p2, nsteps, rho.ctypes.data_as(_dp), work.ctypes.data_as(_dp))
+        lib().orc_run_dv(case.nx, case.ny, case.nz, case.nsub, case.dx, case.dt, pD, pdc, pdv, pvox, pda, na, pv,
+                         pa, p1, p2, nsteps, rho.ctypes.data_as(_dp), work.ctypes.data_as(_dp))
         return rho
     # per step: bulk sources/sinks, then the cell update and the LOD step
     for _ in range(nsteps):
         bulk_sources_sinks(rho, case.dt, *case.bulk)
-        lib().orc_run(case.nx, case.ny, case.nz, case.nsub, case.dx, case.dt, pD, pdc, pdv, pda, na, pv, pa,
-                      p1, p2, 1, rho.ctypes.data_as(_dp), work.ctypes.data_as(_dp))
+        lib().orc_run_dv(case.nx, case.ny, case.nz, case.nsub, case.dx, case.dt, pD, pdc, pdv, pvox, pda, na, pv,
+                         pa, p1, p2, 1, rho.ctypes.data_as(_dp), work.ctypes.data_as(_dp))
     return rho
+
+
+def dirichlet_voxels(case):
+    """dirichlet_value_vectors ([nx][ny][nz][nsub]) after the case's per-voxel Dirichlet nodes
+    (add_dirichlet_node / update_dirichlet_node, BioFVM_microenvironment.cpp:226-290, applied in
+    list order over the per-substrate values), or None when the case has none."""
+    if getattr(case, "dnodes", None) is None:
+        return None
+    from casefile import effective_dirichlet
+    dvalue, _ = effective_dirichlet(case)
+    out = np.empty((case.nx * case.ny * case.nz, case.nsub))
+    out[:] = dvalue
+    voxel, values = case.dnodes
+    for v, val in zip(np.asarray(voxel), np.asarray(values, dtype=np.float64).reshape(-1, case.nsub)):
+        out[int(v)] = val
+    return np.ascontiguousarray(out.reshape(case.nx, case.ny, case.nz, case.nsub))
 
 
 def gradients(rho, dx):

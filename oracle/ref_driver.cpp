@@ -13,7 +13,7 @@
 //   simulate_cell_sources_and_sinks(dt) followed by simulate_diffusion_decay(...).
 //
 // usage: ref_driver <case.bin> <out.bin> [lod|avx] [granularity] [--time] [--mat out.mat]
-//                   [--bulk rates.bin] [--grad grad.bin]
+//                   [--bulk rates.bin] [--grad grad.bin] [--dnodes nodes.bin]
 //        ref_driver_hip <case.bin> <out.bin> [hip|hipres] ...   (BFVMB_PLUGIN build: the
 //        GPU solver in the diffusion_decay_solver slot; hipres keeps the field resident)
 #include <mpi.h>
@@ -90,7 +90,11 @@ int main(int argc, char **argv)
                             // written as [voxel][substrate][3] doubles
     std::string bulk_path;  // --bulk PATH: supply, target, uptake [nvox][nsub] doubles; per step
                             // simulate_bulk_sources_and_sinks runs before the cell update
+    std::string dnodes_path;  // --dnodes PATH: n, voxel[n], values[n][nsub]: per-voxel Dirichlet
+                              // values set through the four public node setters in turn
     for (int i = 3; i + 1 < argc; ++i) {
+        if (std::string(argv[i]) == "--dnodes")
+            dnodes_path = argv[i + 1];
         if (std::string(argv[i]) == "--mat")
             mat_path = argv[i + 1];
         if (std::string(argv[i]) == "--bulk")
@@ -146,6 +150,35 @@ int main(int argc, char **argv)
             continue;  // keep the defaults resize_space leaves (value 100.0, active)
         M.set_substrate_dirichlet_value(d, dval[d]);
         M.set_substrate_dirichlet_activation(d, dact[d] != 0);
+    }
+    if (!dnodes_path.empty()) {
+        // node k goes through add_dirichlet_node(voxel), add_dirichlet_node(x, y, z),
+        // update_dirichlet_node(x, y, z) or update_dirichlet_node(voxel) for k % 4 = 0..3
+        // (BioFVM_microenvironment.cpp:226-290); the last only for a voxel not yet marked, where
+        // it adds the node (the marked case searches dirichlet_indices, which nothing fills)
+        Reader q{fopen(dnodes_path.c_str(), "rb")};
+        if (!q.f) { perror(dnodes_path.c_str()); return 2; }
+        const int n = q.one<int>();
+        std::vector<int> vox(n);
+        std::vector<double> val((size_t)n * nsub);
+        q.get(vox.data(), vox.size());
+        q.get(val.data(), val.size());
+        fclose(q.f);
+        for (int k = 0; k < n; ++k) {
+            std::vector<double> v(val.begin() + (size_t)k * nsub, val.begin() + (size_t)(k + 1) * nsub);
+            const int x = vox[k] / (ny * nz), y = (vox[k] / nz) % ny, z = vox[k] % nz;
+            int route = k % 4;
+            if (route == 3 && M.is_dirichlet_node(vox[k]))
+                route = 2;
+            if (route == 0)
+                M.add_dirichlet_node(vox[k], v);
+            else if (route == 1)
+                M.add_dirichlet_node(x, y, z, v);
+            else if (route == 2)
+                M.update_dirichlet_node(x, y, z, v);
+            else
+                M.update_dirichlet_node(vox[k], v);
+        }
     }
     typedef void (*slot_t)(Microenvironment &, double, int, int, int *, int *, MPI_Comm);
     slot_t lod = diffusion_decay_solver__constant_coefficients_LOD_3D;

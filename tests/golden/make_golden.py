@@ -11,6 +11,7 @@ scalar solver; that is recorded in the fixture.
     python tests/golden/make_golden.py --mat    # MAT v4 fixtures (<name>.mat + <name>.npz)
     python tests/golden/make_golden.py --bulk   # bulk source/sink fixtures
     python tests/golden/make_golden.py --grad   # gradient fixtures (field + gradients)
+    python tests/golden/make_golden.py --dnodes # per-voxel Dirichlet fixtures
 """
 import os
 import sys
@@ -82,6 +83,52 @@ def grad_cases():
     yield "grad_n2_2x3x2x1", cf.make_case(2, 3, 2, 1, seed=23, nagents=3, nsteps=2, snaps=[2])
 
 
+def _face_voxels(case):
+    x, y, z = np.meshgrid(np.arange(case.nx), np.arange(case.ny), np.arange(case.nz), indexing="ij")
+    on = (x == 0) | (x == case.nx - 1) | (y == 0) | (y == case.ny - 1) | (z == 0) | (z == case.nz - 1)
+    return np.flatnonzero(on.reshape(-1)).astype(np.int32)
+
+
+def _dnodes(case, seed, frac, interior, repeats):
+    """Per-voxel Dirichlet nodes: a fraction of the face voxels (random order), some interior
+    voxels (no effect on the step) and some voxels listed twice (the later values win)."""
+    rng = np.random.default_rng(seed)
+    face = _face_voxels(case)
+    pick = rng.permutation(face)[:max(1, int(round(frac * len(face))))]
+    nvox = case.nx * case.ny * case.nz
+    inner = np.setdiff1d(np.arange(nvox, dtype=np.int32), face)
+    pick = np.concatenate([pick, rng.choice(inner, min(interior, len(inner)), replace=False).astype(np.int32)
+                           if len(inner) else np.zeros(0, np.int32)])
+    pick = np.concatenate([pick, rng.choice(pick, repeats)]).astype(np.int32)
+    values = rng.uniform(0.5, 60.0, (len(pick), case.nsub))
+    return pick, values
+
+
+def dnodes_cases():
+    """Cases with per-voxel Dirichlet values set through add_dirichlet_node /
+    update_dirichlet_node (BioFVM_microenvironment.cpp:226-290)."""
+    c = cf.make_case(17, 23, 31, 3, seed=24, nagents=200, nsteps=5, snaps=[1, 5], shared_voxels=True,
+                     dactive=[1, 0, 1])
+    c.dnodes = _dnodes(c, 25, frac=1.0, interior=40, repeats=60)
+    yield "dnodes_17x23x31x3", c
+    c = cf.make_case(12, 9, 14, 2, seed=26, nagents=30, nsteps=3, snaps=[3], dactive=[-1, 1])
+    c.dnodes = _dnodes(c, 27, frac=0.1, interior=10, repeats=5)
+    yield "dnodes_sparse_12x9x14x2", c
+    c = cf.make_case(6, 1, 8, 2, seed=28, nagents=6, nsteps=3, snaps=[3], dactive=[1, 1])
+    c.dnodes = _dnodes(c, 29, frac=0.5, interior=0, repeats=3)
+    yield "dnodes_ny1_6x1x8x2", c
+
+
+def main_dnodes():
+    for name, case in dnodes_cases():
+        with tempfile.TemporaryDirectory() as d:
+            out = cf.run_reference(case, d, solver="lod", granularity=1)
+            avx = cf.run_reference(case, d, solver="avx", granularity=3)
+        avx_equal = all(np.array_equal(out["fields"][s], avx["fields"][s]) for s in case.snaps)
+        path = save(name, case, out, avx_equal)
+        print(f"{name}: {os.path.getsize(path) / 1e3:.0f} kB, {len(case.dnodes[0])} nodes, avx==lod {avx_equal}")
+
+
 def main_grad():
     for name, case in grad_cases():
         with tempfile.TemporaryDirectory() as d:
@@ -119,6 +166,8 @@ def save(name, case, out, avx_equal):
             arrays[k] = a
     if "grad" in out:
         arrays["grad"] = out["grad"]
+    if case.dnodes is not None:
+        arrays["dnodes_voxel"], arrays["dnodes_values"] = case.dnodes
     for s, f in out["fields"].items():
         arrays[f"field_{s}"] = f
     path = os.path.join(HERE, f"{name}.npz")
@@ -149,6 +198,8 @@ def load(name):
            "fields": {int(s): z[f"field_{int(s)}"] for s in z["snaps"]}}
     if "grad" in z.files:
         ref["grad"] = z["grad"]
+    if "dnodes_voxel" in z.files:
+        case.dnodes = (z["dnodes_voxel"], z["dnodes_values"])
     return case, ref
 
 
@@ -158,6 +209,7 @@ NAMES = ["plain_50cube", "aniso_17x23x31x3", "nx1_1x5x7x2", "ny1_2x1x9x4", "nz1_
 MAT_NAMES = ["mat_offset_13x9x11x3", "mat_1x1x2x1"]
 BULK_NAMES = ["bulk_15x12x10x3", "bulk_8x1x9x2"]
 GRAD_NAMES = ["grad_19x14x9x2", "grad_nz1_7x6x1x2", "grad_n2_2x3x2x1"]
+DNODES_NAMES = ["dnodes_17x23x31x3", "dnodes_sparse_12x9x14x2", "dnodes_ny1_6x1x8x2"]
 ALL_NAMES = NAMES + MAT_NAMES
 
 
@@ -174,5 +226,7 @@ if __name__ == "__main__":
         main_bulk()
     elif "--grad" in sys.argv[1:]:
         main_grad()
+    elif "--dnodes" in sys.argv[1:]:
+        main_dnodes()
     else:
         main()

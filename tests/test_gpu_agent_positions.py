@@ -159,3 +159,45 @@ def test_adjoint_after_moves_matches_fresh_agent_list():
         assert np.array_equal(got, want)
     dead = a1 == 0
     assert dead.any() and np.all(out[0][1][dead] == 0.0) and np.any(out[0][1][~dead] != 0.0)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_last_segment_shared_interior_voxel(precision):
+    """Regression for the rebuilt voxel lists: the occupied voxel with the highest device
+    offset (z*nx*ny + y*nx + x) lies inside the grid and holds several agents, and the field is
+    compared right after the source/sink update, before any LOD step could overwrite a face, on
+    the first regrouping and on a second one (with agents leaving the box)."""
+    from biofvmb import LODSolver
+    case = cf.make_case(19, 17, 15, 2, seed=31, nagents=400, nsteps=1)
+    rng = np.random.default_rng(9)
+    dx = case.dx
+    moves = []
+    for last, nlast, nout in (((16, 14, 12), 5, 0), ((15, 14, 12), 3, 40)):
+        p = rng.uniform(0.0, 1.0, (case.nagents, 3)) * np.array([case.nx, case.ny, case.nz - 4]) * dx
+        sel = rng.choice(case.nagents, nlast + nout, replace=False)
+        p[sel[:nlast]] = (np.array(last) + 0.5) * dx
+        p[sel[nlast:]] = -3.0 * dx
+        moves.append(p)
+    voxel, t1, t2 = orc.agent_tables(case)
+    print(f"[gpu] last shared interior segment {precision}", flush=True)
+    s = LODSolver(case.nx, case.ny, case.nz, case.nsub, case.dx, precision)
+    s.set_coefficients(case.D, case.decay, case.dt)
+    s.set_dirichlet(case.dvalue, np.zeros(case.nsub, np.uint8))
+    s.set_agents(voxel, t1, t2)
+    active = np.ones(case.nagents, np.uint8)
+    rho = case.init.copy()
+    s.upload(rho)
+    for p in moves:
+        v, active = orc.update_voxel_indices(case, p, active)
+        s.update_agent_positions(p)
+        orc.sources_sinks(rho, np.maximum(v, 0), active, t1, t2)
+        s.simulate_cell_sources_and_sinks()
+        got = s.download()
+        if precision == "fp64":
+            assert np.array_equal(got, rho)
+        else:
+            r = np.abs(got - rho) / np.maximum(np.abs(rho), 1e-300)
+            assert r.max() <= 1e-6
+        rho = got.astype(np.float64)  # continue from the library's field (FP32: its rounding)
+    assert 0 < int(active.sum()) < case.nagents
+    s.close()

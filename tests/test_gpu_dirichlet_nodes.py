@@ -1,0 +1,156 @@
+"""Per-voxel Dirichlet values (add_dirichlet_node / update_dirichlet_node,
+BioFVM_microenvironment.cpp:226-290, applied on the six faces by apply_dirichlet_conditions,
+:381-484) against the reference's own outputs (tests/golden/dnodes_*.npz, written by the
+compiled reference through all four public node setters): through the C ABI (single context,
+launch shapes, batch member), the C++ host layer, and the reference's own driver with the GPU
+slot.  FP64 bit for bit; FP32 within 1e-5 of the fixture."""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+import casefile as cf
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+import make_golden as golden  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HOST_DRIVER = os.path.join(ROOT, "biofvm-b_amd", "lib", "bfvmb_driver")
+REF_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver_hip")
+
+
+def _solver(case, precision, launch=None):
+    from biofvmb import LODSolver
+    value, active = cf.effective_dirichlet(case)
+    s = LODSolver(case.nx, case.ny, case.nz, case.nsub, case.dx, precision)
+    s.set_coefficients(case.D, case.decay, case.dt)
+    s.set_dirichlet(value, active)
+    s.set_dirichlet_nodes(*case.dnodes)
+    if launch is not None:
+        s.set_launch(-1, *launch)
+    s.upload(case.init)
+    voxel = s.voxel_indices(case.pos, case.bbox[:3])
+    voxel[voxel < 0] = 0
+    if case.with_agents:
+        s.set_agent_rates(voxel, case.volume, case.S, case.U, case.T, case.dt)
+    return s
+
+
+def _run_snapshots(s, case, ref):
+    out, done = {}, 0
+    for step in sorted(ref["fields"]):
+        s.run(step - done) if case.with_agents else s.simulate_diffusion_decay(step - done)
+        done = step
+        out[step] = s.download()
+    return out
+
+
+@pytest.mark.parametrize("launch", [None, (0, 8, 16), (256, 4, 0)])
+@pytest.mark.parametrize("name", golden.DNODES_NAMES)
+def test_capi_fp64_matches_reference(name, launch):
+    case, ref = golden.load(name)
+    print(f"[gpu] dirichlet nodes {name} launch={launch}", flush=True)
+    s = _solver(case, "fp64", launch)
+    got = _run_snapshots(s, case, ref)
+    s.close()
+    for step, f in ref["fields"].items():
+        assert np.array_equal(got[step], f), f"{name} step {step}: {(got[step] != f).sum()} differ"
+
+
+@pytest.mark.parametrize("name", golden.DNODES_NAMES)
+def test_capi_fp32_error(name):
+    case, ref = golden.load(name)
+    print(f"[gpu] dirichlet nodes fp32 {name}", flush=True)
+    s = _solver(case, "fp32")
+    got = _run_snapshots(s, case, ref)
+    s.close()
+    for step, f in ref["fields"].items():
+        r = np.abs(got[step] - f) / np.maximum(np.abs(f), 1e-300)
+        print(f"{name} step {step}: fp32 max rel {r.max():.3e}, rms {np.sqrt(np.mean(r * r)):.3e}")
+        assert r.max() <= 1e-5
+
+
+def test_uniform_values_replace_nodes_and_activation_keeps_them():
+    """set_dirichlet with values goes back to one value per substrate (as
+    set_substrate_dirichlet_value writes every voxel); with value None only the activations
+    change and the node values stay."""
+    import oracle as orc
+    case, ref = golden.load("dnodes_17x23x31x3")
+    value, active = cf.effective_dirichlet(case)
+    print("[gpu] dirichlet nodes: reset and activation-only updates", flush=True)
+    s = _solver(case, "fp64")
+    s.set_dirichlet(value, active)  # nodes dropped
+    s.run(2)
+    nodes = case.dnodes
+    case.dnodes = None
+    assert np.array_equal(s.download(), orc.run(case, nsteps=2))
+    s.close()
+    case.dnodes = nodes
+    s = _solver(case, "fp64")
+    flipped = np.array([1, 1, 0], np.uint8)
+    s.set_dirichlet(None, flipped)  # node values kept, activations changed
+    s.run(2)
+    case.dactive = flipped.astype(np.int32)
+    assert np.array_equal(s.download(), orc.run(case, nsteps=2))
+    s.close()
+
+
+def test_batch_member_with_nodes():
+    import oracle as orc
+    from biofvmb import Batch
+    case, ref = golden.load("dnodes_sparse_12x9x14x2")
+    other = cf.make_case(11, 9, 13, 2, seed=2, nagents=50, nsteps=3)
+    print("[gpu] dirichlet nodes in a batch member", flush=True)
+    b = Batch([(other.nx, other.ny, other.nz, other.nsub), (case.nx, case.ny, case.nz, case.nsub)], case.dx, "fp64")
+    for i, c in enumerate((other, case)):
+        value, active = cf.effective_dirichlet(c)
+        b.set_coefficients(i, c.D, c.decay, c.dt)
+        b.set_dirichlet(i, value, active)
+        b.upload(i, c.init)
+        voxel, t1, t2 = orc.agent_tables(c)
+        b.set_agents(i, voxel, t1, t2)
+    b.set_dirichlet_nodes(1, *case.dnodes)
+    b.run(case.nsteps)
+    assert np.array_equal(b.download(1), ref["fields"][case.nsteps])
+    assert np.array_equal(b.download(0), orc.run(other, nsteps=case.nsteps))
+    b.close()
+
+
+@pytest.mark.parametrize("mode", ["per_call", "fused"])
+@pytest.mark.parametrize("name", golden.DNODES_NAMES)
+def test_host_driver(name, mode):
+    case, ref = golden.load(name)
+    print(f"[gpu] host driver dirichlet nodes {name} {mode}", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        cpath, opath, npath = (os.path.join(d, f) for f in ("case.bin", "out.bin", "nodes.bin"))
+        case.write(cpath)
+        case.write_dnodes(npath)
+        args = [HOST_DRIVER, cpath, opath, "fp64", "--dnodes", npath] + (["--fused"] if mode == "fused" else [])
+        subprocess.run(args, check=True, timeout=120)
+        out = cf.read_output(opath, case)
+    for step, f in ref["fields"].items():
+        assert np.array_equal(out["fields"][step], f), f"{name} step {step}"
+
+
+@pytest.mark.skipif(not os.path.exists(REF_DRIVER), reason="reference build with the plugin (make -C oracle ref-hip) absent")
+@pytest.mark.parametrize("mode", ["hip", "hipres"])
+@pytest.mark.parametrize("name", golden.DNODES_NAMES)
+def test_reference_driver_with_gpu_slot(name, mode):
+    """The reference's driver sets the nodes through its own Microenvironment; the plugin reads
+    dirichlet_value_vectors and hands the face values to the library."""
+    case, ref = golden.load(name)
+    print(f"[gpu] reference driver + plugin dirichlet nodes {name} {mode}", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        cpath, opath, npath = (os.path.join(d, f) for f in ("case.bin", "out.bin", "nodes.bin"))
+        case.write(cpath)
+        case.write_dnodes(npath)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([REF_DRIVER, cpath, opath, mode, "1", "--dnodes", npath], check=True, timeout=120, env=env)
+        out = cf.read_output(opath, case)
+    for step, f in ref["fields"].items():
+        assert np.array_equal(out["fields"][step], f), f"{name} step {step}"

@@ -217,3 +217,25 @@ def test_fp64_tiny_numerators_bitwise(name, launch):
     if not np.array_equal(got, ref):
         bad = np.argwhere(got != ref)
         pytest.fail(f"{name}: {len(bad)} values differ, first {bad[:5].tolist()}")
+
+
+@pytest.mark.parametrize("shape", [(40, 36, 30, 2), (700, 5, 6, 1), (5, 6, 700, 1)])
+def test_fp64_nonfinite_values_as_the_reference(shape):
+    """Infinite values in the field: the reference's division gives +-inf (and inf - inf in
+    the back substitution NaN), which the corrected reciprocal alone would turn into NaN
+    earlier; the sweeps take the exact-division path for such lines.  Finite values bit for
+    bit against the oracle, NaNs at the same places (NaN payloads are not compared)."""
+    nx, ny, nz, nsub = shape
+    rng = np.random.default_rng(41)
+    init = rng.lognormal(0.0, 0.5, (nx, ny, nz, nsub))
+    idx = tuple(rng.integers(1, max(2, n - 1), 6) for n in (nx, ny, nz)) + (rng.integers(0, nsub, 6),)
+    init[idx] = np.where(np.arange(6) % 2 == 0, np.inf, -np.inf)
+    case = cf.make_case(nx, ny, nz, nsub, seed=41, nagents=0, nsteps=2, init=init,
+                        dvalue=[38.0] + [0.0] * (nsub - 1), dactive=[1] + [0] * (nsub - 1), with_agents=False)
+    print(f"[gpu] non-finite values {shape}", flush=True)
+    ref = orc.run(case)
+    assert np.isinf(ref).any()
+    got = gpu_run(case, "fp64")
+    assert np.array_equal(np.isnan(got), np.isnan(ref))
+    fin = ~np.isnan(ref)
+    assert np.array_equal(got[fin], ref[fin])

@@ -20,7 +20,7 @@ import make_golden as golden  # noqa: E402
 
 
 # ---------------------------------------------------------------- 1. reference fixtures
-@pytest.mark.parametrize("name", golden.ALL_NAMES)
+@pytest.mark.parametrize("name", golden.ALL_NAMES + golden.DNODES_NAMES)
 def test_oracle_matches_reference_bitwise(name):
     case, ref = golden.load(name)
     voxel, t1, t2 = orc.agent_tables(case)
@@ -32,10 +32,51 @@ def test_oracle_matches_reference_bitwise(name):
         assert np.array_equal(got[s], f), f"{name} step {s}: {(got[s] != f).sum()} values differ"
 
 
-@pytest.mark.parametrize("name", golden.ALL_NAMES)
+@pytest.mark.parametrize("name", golden.ALL_NAMES + golden.DNODES_NAMES)
 def test_reference_avx_solver_equals_scalar_solver(name):
     _, ref = golden.load(name)
     assert ref["avx_equal"]
+
+
+@pytest.mark.parametrize("name", golden.DNODES_NAMES)
+def test_dirichlet_node_fixtures_discriminate(name):
+    """The per-voxel fixtures pin the node values: the same case with one value per substrate
+    (nodes dropped) or with the nodes' values read at the wrong voxel (x and z swapped in the
+    voxel index) differs from the reference's output."""
+    case, ref = golden.load(name)
+    last = max(ref["fields"])
+    voxel, values = case.dnodes
+    case.dnodes = None
+    assert not np.array_equal(orc.run(case, nsteps=last), ref["fields"][last])
+    x, rem = np.divmod(np.asarray(voxel), case.ny * case.nz)
+    y, z = np.divmod(rem, case.nz)
+    if case.nx == case.nz:
+        pytest.skip("x/z swap is the identity map on this grid")
+    ok = (z < case.nx) & (x < case.nz)
+    case.dnodes = (((z * case.ny + y) * case.nz + x)[ok], np.asarray(values)[ok])
+    assert not np.array_equal(orc.run(case, nsteps=last), ref["fields"][last])
+
+
+@pytest.mark.parametrize("name", golden.DNODES_NAMES)
+def test_dirichlet_nodes_fix_every_active_face_voxel(name):
+    """Mathematics of apply_dirichlet_conditions: after a step, every face voxel of an active
+    substrate holds exactly its node value (or the per-substrate value), interior nodes leave
+    the field unchanged against the same case with them removed."""
+    case, ref = golden.load(name)
+    value, active = cf.effective_dirichlet(case)
+    out = orc.run(case, nsteps=1)
+    dv = orc.dirichlet_voxels(case)
+    face = np.zeros((case.nx, case.ny, case.nz), bool)
+    face[0] = face[-1] = True
+    face[:, 0] = face[:, -1] = True
+    face[:, :, 0] = face[:, :, -1] = True
+    for d in range(case.nsub):
+        if active[d]:
+            assert np.array_equal(out[..., d][face], dv[..., d][face])
+    voxel, values = case.dnodes
+    on_face = face.reshape(-1)[np.asarray(voxel)]
+    case.dnodes = (np.asarray(voxel)[on_face], np.asarray(values)[on_face])
+    assert np.array_equal(orc.run(case, nsteps=1), out)
 
 
 # ---------------------------------------------------------------- 2. mathematics
