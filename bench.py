@@ -41,19 +41,8 @@ def log(msg):
 
 
 def make_inputs(n=N, nagents=N_AGENTS, seed=2024):
-    rng = np.random.default_rng(seed)
-    init = rng.lognormal(0.0, 0.5, (n, n, n, NSUB))
-    center = n * DX / 2.0
-    radius = 3000.0 * (n / N)
-    d = rng.normal(size=(nagents, 3))
-    d /= np.linalg.norm(d, axis=1, keepdims=True)
-    r = radius * rng.uniform(0, 1, nagents) ** (1.0 / 3.0)
-    pos = center + d * r[:, None]
-    S = rng.uniform(0, 5, (nagents, NSUB))
-    U = rng.uniform(0, 20, (nagents, NSUB))
-    T = np.ones((nagents, NSUB))
-    vol = np.full(nagents, 4.0 / 3.0 * np.pi * 125.0)
-    return init, pos, vol, S, U, T
+    from biofvmb import workloads
+    return workloads.dropin_inputs(n=n, nagents=nagents, seed=seed)
 
 
 def setup_solver(precision, init, pos, vol, S, U, T, stream):
@@ -100,34 +89,20 @@ def bench_batched(torch, stream, steps=20, seed=3):
     U{1..8}, D ~ logU[1e2, 1e5], decay ~ logU[1e-3, 1], U{0..50000} agents each; the
     autotuner picks the launch shapes, then `steps` steps run as one captured graph, in FP64
     and in FP32 storage."""
-    from biofvmb import Batch
-    rng = np.random.default_rng(seed)
-    m = 64
-    shapes = [tuple(int(v) for v in rng.integers(32, 257, 3)) + (int(rng.integers(1, 9)),) for _ in range(m)]
-    params = []
-    for (nx, ny, nz, ns) in shapes:
-        Dm = 10.0 ** rng.uniform(2, 5, ns)
-        lam = 10.0 ** rng.uniform(-3, 0, ns)
-        na = int(rng.integers(0, 50001))
-        ext = np.array([nx, ny, nz]) * DX
-        pos = rng.uniform(0, 1, (na, 3)) * ext
-        params.append((Dm, lam, pos, rng.uniform(0, 5, (na, ns)), rng.uniform(0, 20, (na, ns))))
+    from biofvmb import Batch, workloads
+    members = workloads.batch_members(seed=seed)
+    shapes = [mb["shape"] for mb in members]
+    m = len(members)
     total_vs = sum(nx * ny * nz * ns for nx, ny, nz, ns in shapes)
     out = {"members": m, "voxel_substrates": total_vs, "steps": steps}
     for prec in ("fp64", "fp32"):
         b = Batch(shapes, DX, prec)
         b.set_stream(stream)
-        for i, ((nx, ny, nz, ns), (Dm, lam, pos, S, U)) in enumerate(zip(shapes, params)):
-            b.set_coefficients(i, Dm, lam, DT)
-            b.set_dirichlet(i, np.full(ns, 10.0), np.arange(ns) % 2 == 0)
-            b.upload(i, np.full((nx, ny, nz, ns), 1.0 + 0.01 * i))
-            from biofvmb.solver import voxel_indices
-            vox = voxel_indices((nx, ny, nz), DX, pos)
-            vox[vox < 0] = 0
-            vol = 4.0 / 3.0 * np.pi * 125.0
-            k = (DT * vol) / (DX * DX * DX)
-            t1 = (S * 1.0) * k
-            t2 = 1.0 + k * U + k * S
+        for i, mb in enumerate(members):
+            dvalue, dactive, init, vox, t1, t2 = workloads.batch_member_setup(i, mb)
+            b.set_coefficients(i, mb["D"], mb["decay"], DT)
+            b.set_dirichlet(i, dvalue, dactive)
+            b.upload(i, init)
             b.set_agents(i, vox, t1, t2)
         chosen = b.autotune(reps=2)
         b.run(steps, graph=True)  # capture + warm-up
@@ -149,23 +124,19 @@ def bench_batched(torch, stream, steps=20, seed=3):
 def bench_c5(torch, stream, steps=10, nagents=10_000_000, seed=9):
     """FP32 bulk path at 640^3 x 8 substrates (log-spaced D and decay, Dirichlet on
     substrates 0-1, 10M agents in a centred 5 mm sphere), constant initial field."""
-    from biofvmb import LODSolver
-    n, ns = 640, 8
-    rng = np.random.default_rng(seed)
-    Dm = np.logspace(2, 5, ns)
-    lam = np.logspace(-3, 0, ns)
+    from biofvmb import LODSolver, workloads
+    n, ns = workloads.LARGE["n"], workloads.LARGE["nsub"]
+    Dm, lam, dvalue, dactive, pos, S, U = workloads.large_inputs(nagents=nagents, seed=seed)
     s = LODSolver(n, n, n, ns, DX, "fp32")
     s.set_stream(stream)
     s.set_coefficients(Dm, lam, DT)
-    s.set_dirichlet(np.full(ns, 38.0), (np.arange(ns) < 2).astype(np.uint8))
-    s.upload(np.ones((n, n, n, ns)))
-    d = rng.normal(size=(nagents, 3))
-    d /= np.linalg.norm(d, axis=1, keepdims=True)
-    pos = n * DX / 2.0 + d * (5000.0 * rng.uniform(0, 1, nagents) ** (1.0 / 3.0))[:, None]
+    s.set_dirichlet(dvalue, dactive)
+    ones = torch.ones((n, n, n, ns), dtype=torch.float64, device="cuda:0")  # no 16.8 GB host array
+    s.upload(ones)
+    del ones
     vox = s.voxel_indices(pos)
     vox[vox < 0] = 0
-    s.set_agent_rates(vox, np.full(nagents, 4.0 / 3.0 * np.pi * 125.0), rng.uniform(0, 5, (nagents, ns)),
-                      rng.uniform(0, 20, (nagents, ns)), np.ones((nagents, ns)), DT)
+    s.set_agent_rates(vox, np.full(nagents, workloads.AGENT_VOLUME), S, U, np.ones((nagents, ns)), DT)
     gpu_s, wall = time_steps(s, torch, steps, 2)
     s.close()
     vs = n ** 3 * ns * steps / gpu_s

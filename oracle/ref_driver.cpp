@@ -13,7 +13,7 @@
 //   simulate_cell_sources_and_sinks(dt) followed by simulate_diffusion_decay(...).
 //
 // usage: ref_driver <case.bin> <out.bin> [lod|avx] [granularity] [--time] [--mat out.mat]
-//                   [--bulk rates.bin] [--grad grad.bin] [--dnodes nodes.bin]
+//                   [--bulk rates.bin] [--grad grad.bin] [--dnodes nodes.bin] [--serial-agents]
 //        ref_driver_hip <case.bin> <out.bin> [hip|hipres] ...   (BFVMB_PLUGIN build: the
 //        GPU solver in the diffusion_decay_solver slot; hipres keeps the field resident)
 #include <mpi.h>
@@ -90,6 +90,11 @@ int main(int argc, char **argv)
                             // written as [voxel][substrate][3] doubles
     std::string bulk_path;  // --bulk PATH: supply, target, uptake [nvox][nsub] doubles; per step
                             // simulate_bulk_sources_and_sinks runs before the cell update
+    bool serial_agents = false;  // --serial-agents: simulate_cell_sources_and_sinks on one thread (its
+                                 // omp loop races where agents share a voxel), the LOD on all
+    for (int i = 3; i < argc; ++i)
+        if (std::string(argv[i]) == "--serial-agents")
+            serial_agents = true;
     std::string dnodes_path;  // --dnodes PATH: n, voxel[n], values[n][nsub]: per-voxel Dirichlet
                               // values set through the four public node setters in turn
     for (int i = 3; i + 1 < argc; ++i) {
@@ -209,6 +214,7 @@ int main(int argc, char **argv)
         M.bulk_uptake_rate_function = bulk_uptake;
     }
     std::copy(init.begin(), init.end(), M.p_density_vectors->begin());
+    std::vector<double>().swap(init);  // large cases: keep only the library's copy
 
     std::vector<int> voxel(nagents);
     std::vector<double> t1((size_t)nagents * nsub), t2((size_t)nagents * nsub);
@@ -259,7 +265,13 @@ int main(int argc, char **argv)
                 bfvmb_hip_cell_sources_and_sinks(M, all_basic_agents, dt);
             else
 #endif
+            {
+                const int threads = omp_get_max_threads();
+                if (serial_agents)
+                    omp_set_num_threads(1);
                 M.simulate_cell_sources_and_sinks(dt);
+                omp_set_num_threads(threads);
+            }
         }
 #ifdef BFVMB_PLUGIN
         if (solver == "hipres" && s == 1)

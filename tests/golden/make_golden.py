@@ -12,6 +12,9 @@ scalar solver; that is recorded in the fixture.
     python tests/golden/make_golden.py --bulk   # bulk source/sink fixtures
     python tests/golden/make_golden.py --grad   # gradient fixtures (field + gradients)
     python tests/golden/make_golden.py --dnodes # per-voxel Dirichlet fixtures
+    python tests/golden/make_golden.py --dropin # the drop-in workload at full size (400^3 x 4,
+                                                # 1M agents, 200 steps; ~50 GB of memory): sampled
+                                                # voxels and whole-field checksums at steps 1, 10, 200
 """
 import os
 import sys
@@ -129,6 +132,77 @@ def main_dnodes():
         print(f"{name}: {os.path.getsize(path) / 1e3:.0f} kB, {len(case.dnodes[0])} nodes, avx==lod {avx_equal}")
 
 
+DROPIN_SNAPS = [1, 10, 200]
+DROPIN_SAMPLES = 4096
+
+
+def dropin_case():
+    """The drop-in workload (biofvmb.workloads.dropin_inputs, seed 2024) as a case."""
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(HERE)), "biofvm-b_amd"))
+    from biofvmb import workloads as wl
+    w = wl.DROPIN
+    init, pos, vol, S, U, T = wl.dropin_inputs()
+    return cf.Case(w["n"], w["n"], w["n"], w["nsub"], wl.DX, wl.DT, np.array(w["D"]), np.array(w["decay"]),
+                   np.array(w["dvalue"]), np.array(w["dactive"]), init, pos, vol, S, U, T, DROPIN_SNAPS[-1],
+                   list(DROPIN_SNAPS), True)
+
+
+def dropin_sample_voxels(nvox):
+    return np.sort(np.random.default_rng(12345).choice(nvox, DROPIN_SAMPLES, replace=False))
+
+
+def main_dropin(threads=8):
+    """Runs the reference on the full drop-in workload (the cell update on one thread, the LOD
+    step on `threads`) and keeps, per snapshot, the fields at DROPIN_SAMPLES seeded voxels and
+    whole-field checksums per substrate (biofvmb.workloads.field_checksums), plus checksums of
+    the agent mapping and constants."""
+    import gc
+    import subprocess
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(HERE)), "biofvm-b_amd"))
+    from biofvmb import workloads as wl
+    with tempfile.TemporaryDirectory(dir=os.environ.get("BFVMB_SCRATCH")) as d:
+        cpath, opath = os.path.join(d, "case.bin"), os.path.join(d, "out.bin")
+        case = dropin_case()
+        case.write(cpath)
+        n, ns, na = case.nx, case.nsub, case.nagents
+        del case
+        gc.collect()
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        subprocess.run([cf.REF_DRIVER, cpath, opath, "lod", "1", "--serial-agents"], env=env, check=True)
+        os.remove(cpath)
+        nvox = n ** 3
+        idx = dropin_sample_voxels(nvox)
+        arrays = dict(sample_voxels=idx, snaps=np.array(DROPIN_SNAPS))
+        with open(opath, "rb") as f:
+            assert f.read(8) == cf.MAGIC_OUT
+            assert int(np.fromfile(f, np.int32, 1)[0]) == na
+            voxel = np.fromfile(f, np.int32, na)
+            t1 = np.fromfile(f, np.float64, na * ns).reshape(na, ns)
+            t2 = np.fromfile(f, np.float64, na * ns).reshape(na, ns)
+            arrays["agent_voxel_hash"] = np.array(wl.field_checksums(voxel.astype(np.float64)[:, None])[0][3],
+                                                  dtype=np.uint64)
+            arrays["temp_checksums"] = np.array([[c[3] for c in wl.field_checksums(t)] for t in (t1, t2)],
+                                                dtype=np.uint64)
+            del voxel, t1, t2
+            nsnap = int(np.fromfile(f, np.int32, 1)[0])
+            for _ in range(nsnap):
+                step = int(np.fromfile(f, np.int32, 1)[0])
+                rho = np.fromfile(f, np.float64, nvox * ns).reshape(nvox, ns)
+                arrays[f"samples_{step}"] = rho[idx]
+                cs = wl.field_checksums(rho)
+                arrays[f"sums_{step}"] = np.array([c[:3] for c in cs])
+                arrays[f"hash_{step}"] = np.array([c[3] for c in cs], dtype=np.uint64)
+                print(f"step {step}: sums {arrays[f'sums_{step}'][:, 0]}", flush=True)
+                del rho
+    path = os.path.join(HERE, "dropin_400cube_x4.npz")
+    np.savez_compressed(path, **arrays)
+    print(f"{path}: {os.path.getsize(path) / 1e3:.0f} kB")
+
+
+def load_dropin():
+    return dict(np.load(os.path.join(HERE, "dropin_400cube_x4.npz"), allow_pickle=False))
+
+
 def main_grad():
     for name, case in grad_cases():
         with tempfile.TemporaryDirectory() as d:
@@ -228,5 +302,7 @@ if __name__ == "__main__":
         main_grad()
     elif "--dnodes" in sys.argv[1:]:
         main_dnodes()
+    elif "--dropin" in sys.argv[1:]:
+        main_dropin()
     else:
         main()
