@@ -224,6 +224,41 @@ def bench_agent_moves(torch, stream, n=400, nagents=1_000_000, reps=10, seed=11)
     return out
 
 
+def bench_per_call(steps=20, precision="fp64"):
+    """The drop-in workload's shape (400^3 x 4, 1M agents in the 3 mm sphere) through the C++
+    host layer (biofvmb::Microenvironment, no reference code), one step per call as the
+    reference's drivers call the slots (simulate_cell_sources_and_sinks, then
+    simulate_diffusion_decay), against one run(agents, dt, steps) call; a separate process
+    (biofvm-b_amd/lib/bfvmb_driver --bench-dropin)."""
+    exe = os.path.join(ROOT, "biofvm-b_amd", "lib", "bfvmb_driver")
+    out = subprocess.run([exe, "--bench-dropin", str(N), str(N_AGENTS), str(steps), precision], capture_output=True,
+                         text=True, check=True, timeout=600)
+    rec = json.loads(out.stdout.strip().splitlines()[-1])
+    rec["value"] = N ** 3 * NSUB / (rec["per_call_ms_per_step"] / 1e3)
+    rec["unit"] = "voxel-substrate updates/s, one step per call (inputs drawn with std::mt19937_64)"
+    log(f"per-call host layer: {rec['per_call_ms_per_step']:.3f} ms/step, run(): {rec['run_ms_per_step']:.3f} ms/step")
+    return rec
+
+
+def traffic_record(dom):
+    """HBM bytes per launch of kernel `dom` from the FETCH_SIZE / WRITE_SIZE profile in
+    profiles/, with the sha256 of the sweep source it was measured on; null when the source
+    has changed since (the counters then describe other kernels)."""
+    import hashlib
+    tpath = os.path.join(ROOT, "profiles", "traffic_400cube_fp64.json")
+    if not os.path.exists(tpath):
+        return None, None
+    rec = json.load(open(tpath))
+    src = os.path.join(ROOT, "biofvm-b_amd", "csrc", "lod.hip")
+    cur = hashlib.sha256(open(src, "rb").read()).hexdigest()
+    meta = {"profile": "profiles/traffic_400cube_fp64.json", "lod_hip_sha256": rec.get("lod_hip_sha256"),
+            "current_lod_hip_sha256": cur, "commit": rec.get("commit")}
+    if rec.get("lod_hip_sha256") != cur:
+        meta["stale"] = True
+        return None, meta
+    return rec.get("bytes_per_launch", {}).get(dom), meta
+
+
 def cpu_baseline(repeats=3):
     """The reference (oracle/_ref) on a bounded sample: 200^3 x 4 with 125k agents (1/8 of
     the domain, same densities), 16 steps, median of three runs (≈20 s of CPU work on 16
@@ -310,13 +345,10 @@ def main():
     dom = max(("sweep_x", "sweep_y", "sweep_z"), key=lambda k: f64["phases_ms"][k])
     bytes_per_launch = N ** 3 * NSUB * 2 * 8
     achieved = bytes_per_launch / (f64["phases_ms"][dom] / 1e3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_400cube_fp64.json")
-    if os.path.exists(tpath):  # HBM bytes per launch from FETCH_SIZE + WRITE_SIZE counters (profiles/)
-        traffic = json.load(open(tpath)).get("bytes_per_launch", {}).get(dom)
+    traffic, traffic_meta = traffic_record(dom)  # FETCH_SIZE + WRITE_SIZE counters (profiles/)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "algorithmic_bytes_per_launch": bytes_per_launch}
+                "algorithmic_bytes_per_launch": bytes_per_launch, "traffic_measurement": traffic_meta}
     step_gbs = N ** 3 * NSUB * 48 / (f64["ms_per_step"] / 1e3) / 1e9
     extra = {}
     if not args.no_extra:
@@ -324,6 +356,7 @@ def main():
         extra["batched"]["fp32_vs_fp64_dropin"] = extra["batched"]["fp32"]["value"] / f64["value"]
         extra["adjoint"] = bench_adjoint(torch, stream)
         extra["agent_moves"] = bench_agent_moves(torch, stream)
+        extra["per_call_host_layer"] = bench_per_call()
     if args.c5:
         extra["fp32_640cube_x8"] = bench_c5(torch, stream)
     cpu = None if args.no_cpu else cpu_baseline()

@@ -80,6 +80,8 @@ SIGNATURES = {
     "bfvmb_download_density_device": (_c.c_int, [_vp, _vp]),
     "bfvmb_adjoint_backward_device": (_c.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "bfvmb_batch_update_agent_positions": (_c.c_int, [_vp, _c.c_int, _c.c_int, _vp, _dp, _c.c_int]),
+    "bfvmb_assign_agent_positions": (_c.c_int, [_vp, _c.c_int, _vp, _dp, _c.c_int]),
+    "bfvmb_batch_assign_agent_positions": (_c.c_int, [_vp, _c.c_int, _c.c_int, _vp, _dp, _c.c_int]),
 }
 
 _lib = None
@@ -105,12 +107,32 @@ def load(path: str = LIB_PATH):
         except ImportError:
             pass
         lib = _c.CDLL(path)
+        missing = []
         for name, (res, args) in SIGNATURES.items():
-            fn = getattr(lib, name)
+            try:
+                fn = getattr(lib, name)
+            except AttributeError:
+                missing.append(name)
+                continue
             fn.restype = res
             fn.argtypes = args
-        _lib = lib
+        _lib = lib if not missing else _Partial(lib, path, missing)
     return _lib
+
+
+class _Partial:
+    """A library built from older sources than this binding: the entry points it lacks raise
+    when called (instead of failing every use of the library at load time)."""
+
+    def __init__(self, lib, path, missing):
+        self._lib, self._path, self._missing = lib, path, set(missing)
+
+    def __getattr__(self, name):
+        if name in self._missing:
+            def fail(*args, **kwargs):
+                raise BioFVMBError(f"{self._path} has no {name}: rebuild it (make -C biofvm-b_amd)")
+            return fail
+        return getattr(self._lib, name)
 
 
 def dptr(a: np.ndarray):

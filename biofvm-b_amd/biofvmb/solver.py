@@ -192,6 +192,16 @@ class LODSolver:
                     "update_agent_positions")
         del keep
 
+    def assign_agent_positions(self, pos, bbox_min=(0.0, 0.0, 0.0)) -> None:
+        """The same move by assign_position's rule: agents whose new position lies outside the
+        bounding box keep their voxel and stay active (bfvmb_assign_agent_positions)."""
+        n = getattr(self, "_nagents", 0)
+        ptr, dev, keep = _positions(pos, n)
+        lo = np.ascontiguousarray(bbox_min, dtype=np.float64)
+        self._check(self._lib.bfvmb_assign_agent_positions(self._h, n, ptr, L.dptr(lo), dev),
+                    "assign_agent_positions")
+        del keep
+
     def compute_gradients(self) -> None:
         """Gradients of the current field into the device buffer (bfvmb_compute_gradients)."""
         self._check(self._lib.bfvmb_compute_gradients(self._h), "compute_gradients")
@@ -350,6 +360,14 @@ class Batch:
         lo = np.ascontiguousarray(bbox_min, dtype=np.float64)
         self._check(self._lib.bfvmb_batch_update_agent_positions(self._h, i, n, ptr, L.dptr(lo), dev),
                     "batch_update_agent_positions")
+        del keep
+
+    def assign_agent_positions(self, i, pos, n, bbox_min=(0.0, 0.0, 0.0)):
+        """bfvmb_batch_assign_agent_positions for member i (n = length of its agent list)."""
+        ptr, dev, keep = _positions(pos, n)
+        lo = np.ascontiguousarray(bbox_min, dtype=np.float64)
+        self._check(self._lib.bfvmb_batch_assign_agent_positions(self._h, i, n, ptr, L.dptr(lo), dev),
+                    "batch_assign_agent_positions")
         del keep
 
     def set_bulk_rates(self, i, supply=None, target=None, uptake=None):

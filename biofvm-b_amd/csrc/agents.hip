@@ -1,17 +1,19 @@
 // agents.hip -- moving agents: the voxel of every agent from its position, and the
 // regrouping of the source/sink list by voxel, on the device.
 //
-// Basic_Agent::update_voxel_index (BioFVM_basic_agent.cpp:137-153) with
-// Cartesian_Mesh::nearest_voxel_local_index (BioFVM_parallel.cpp:418-522, world size 1):
-//   outside the bounding box (BioFVM_mesh.cpp:204-213)  -> the agent becomes inactive;
-//   inside -> floor((p - min)/dx) per axis, clamped into the grid, voxel x*ny*nz + y*nz + z,
-//             clamped into [0, nvox).
-// An inactive agent stays inactive (the reference never sets is_active again).  The agents
-// active when the list was set form a compact list (ascending caller index); each update
-// sorts them stably by device offset z*nx*ny + y*nx + x (a radix sort keeps equal keys in
-// input order, so agents sharing a voxel stay in list order, as the reference's loop applies
-// them), dead agents last, then rebuilds the CSR of occupied voxels and reorders
-// temp1/temp2.  The caller's arithmetic for the mapping is repeated operation for operation
+// The two ways the reference moves an agent (world size 1):
+//  * position set, then Basic_Agent::update_voxel_index (BioFVM_basic_agent.cpp:137-153): a
+//    position outside the bounding box (BioFVM_mesh.cpp:204-213) deactivates the agent;
+//  * Basic_Agent::assign_position (BioFVM_basic_agent.cpp:112-128): a position outside the box
+//    is rejected, the agent keeps its voxel and stays active (keep_outside).
+// Inside the box both give Cartesian_Mesh::nearest_voxel_local_index (BioFVM_parallel.cpp:
+// 418-522): floor((p - min)/dx) per axis, clamped into the grid, voxel x*ny*nz + y*nz + z,
+// clamped into [0, nvox).  An inactive agent stays inactive (the reference never sets
+// is_active again).  The agents active when the list was set form a compact list (ascending
+// caller index); each update sorts them stably by device offset z*nx*ny + y*nx + x (64-bit
+// keys; a radix sort keeps equal keys in input order, so agents sharing a voxel stay in list
+// order, as the reference's loop applies them), dead agents last, then rebuilds the CSR of
+// occupied voxels and reorders temp1/temp2.  The caller's arithmetic for the mapping is repeated operation for operation
 // (bfvmb_voxel_indices on the host), so both give the same voxels.
 #include <algorithm>
 
@@ -25,10 +27,11 @@ struct AgentRemap {
     int n0 = 0;                          // agents active when the list was set
     int *ids = nullptr;                  // [n0] caller index of compact agent i (ascending)
     uint8_t *alive = nullptr;            // [n0] still active
+    unsigned long long *key = nullptr;   // [n0] device offset of each compact agent's current voxel
     double *tab = nullptr;               // [n0][nsub][2] temp1, temp2 in compact order (agent-major:
                                          // the regrouping gathers whole agents)
     double *pos = nullptr;               // [given][3] staging of host positions
-    unsigned *kin = nullptr, *kout = nullptr;  // [n0] sort keys (device offset, nvox = dead)
+    unsigned long long *kin = nullptr, *kout = nullptr;  // [n0] sort keys (device offset, nvox = dead)
     int *vin = nullptr, *vout = nullptr;       // [n0] compact indices
     int *head = nullptr, *sid = nullptr;       // [n0] segment heads, segment ids
     int *counts = nullptr;               // [2] segments, agents alive
@@ -45,7 +48,7 @@ void free_remap(Domain &d)
     AgentRemap *r = d.remap;
     if (!r)
         return;
-    for (void *p : {(void *)r->ids, (void *)r->alive, (void *)r->tab, (void *)r->pos, (void *)r->kin,
+    for (void *p : {(void *)r->ids, (void *)r->alive, (void *)r->key, (void *)r->tab, (void *)r->pos, (void *)r->kin,
                     (void *)r->kout, (void *)r->vin, (void *)r->vout, (void *)r->head, (void *)r->sid,
                     (void *)r->counts, r->tmp, (void *)r->seg_start_new, (void *)r->seg_off_new})
         (void)hipFree(p);
@@ -62,47 +65,54 @@ struct MapGrid {
     int nx, ny, nz;
 };
 
-// sort key of compact agent i: its voxel's device offset, or nvox once it is inactive
+// sort key of compact agent i: its voxel's device offset, or nvox once it is inactive.  A
+// position outside the box deactivates the agent, or with keep_outside leaves it where it was.
 __global__ __launch_bounds__(256) void k_agent_keys(const double *__restrict__ pos, const int *__restrict__ ids,
-                                                    uint8_t *__restrict__ alive, int n0, MapGrid m,
-                                                    unsigned *__restrict__ key, int *__restrict__ val)
+                                                    uint8_t *__restrict__ alive, unsigned long long *__restrict__ cur,
+                                                    int n0, MapGrid m, int keep_outside,
+                                                    unsigned long long *__restrict__ key, int *__restrict__ val)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n0)
         return;
     const double *p = pos + 3 * (size_t)ids[i];
-    bool live = alive[i] != 0;
+    bool inside = true;
     long long c[3];
     for (int a = 0; a < 3; ++a) {
         const double v = p[a];
         if (!(v >= m.lo[a] && v <= m.hi[a]))  // outside the bounding box (or not a number)
-            live = false;
-        long long q = live ? (long long)floor((v - m.lo[a]) / m.dx) : 0;
+            inside = false;
+        long long q = inside ? (long long)floor((v - m.lo[a]) / m.dx) : 0;
         if (q >= m.g[a])
             q = m.g[a] - 1;
         if (q < 0)
             q = 0;
         c[a] = q;
     }
-    unsigned k = (unsigned)m.nv;
-    if (live) {
-        long long v = c[0] * m.g[1] * m.g[2] + c[1] * m.g[2] + c[2];  // reference numbering
-        if (v < 0)
-            v = 0;
-        if (v >= m.nv)
-            v = m.nv - 1;
-        const long long nyz = (long long)m.ny * m.nz;
-        const long long x = v / nyz, rem = v - x * nyz, y = rem / m.nz, z = rem - y * m.nz;
-        k = (unsigned)((z * m.ny + y) * m.nx + x);
-    } else {
-        alive[i] = 0;
+    unsigned long long k = (unsigned long long)m.nv;
+    if (alive[i]) {
+        if (inside) {
+            long long v = c[0] * m.g[1] * m.g[2] + c[1] * m.g[2] + c[2];  // reference numbering
+            if (v < 0)
+                v = 0;
+            if (v >= m.nv)
+                v = m.nv - 1;
+            const long long nyz = (long long)m.ny * m.nz;
+            const long long x = v / nyz, rem = v - x * nyz, y = rem / m.nz, z = rem - y * m.nz;
+            k = (unsigned long long)((z * m.ny + y) * m.nx + x);
+            cur[i] = k;
+        } else if (keep_outside) {
+            k = cur[i];
+        } else {
+            alive[i] = 0;
+        }
     }
     key[i] = k;
     val[i] = i;
 }
 
-__global__ __launch_bounds__(256) void k_agent_heads(const unsigned *__restrict__ key, int n0, unsigned dead,
-                                                     int *__restrict__ head)
+__global__ __launch_bounds__(256) void k_agent_heads(const unsigned long long *__restrict__ key, int n0,
+                                                     unsigned long long dead, int *__restrict__ head)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n0)
@@ -112,9 +122,10 @@ __global__ __launch_bounds__(256) void k_agent_heads(const unsigned *__restrict_
 
 // CSR of occupied voxels and the agent tables in sorted order (dead agents sit after the
 // last segment, in no segment)
-__global__ __launch_bounds__(256) void k_agent_build(const unsigned *__restrict__ key, const int *__restrict__ val,
-                                                     const int *__restrict__ head, const int *__restrict__ sid,
-                                                     int n0, int nsub, unsigned dead,
+__global__ __launch_bounds__(256) void k_agent_build(const unsigned long long *__restrict__ key,
+                                                     const int *__restrict__ val, const int *__restrict__ head,
+                                                     const int *__restrict__ sid, int n0, int nsub,
+                                                     unsigned long long dead,
                                                      const double *__restrict__ tab,
                                                      int *__restrict__ seg_start, long long *__restrict__ seg_off,
                                                      double *__restrict__ t1, double *__restrict__ t2,
@@ -162,7 +173,7 @@ __global__ __launch_bounds__(256) void k_agent_unsort(const double *__restrict__
 int bits_for(unsigned long long v)
 {
     int b = 1;
-    while (b < 32 && (v >> b) != 0)
+    while (b < 64 && (v >> b) != 0)
         ++b;
     return b;
 }
@@ -179,17 +190,21 @@ int remap_setup(Group *g, Domain &d)
     r->host_ids = d.agent_order;
     std::sort(r->host_ids.begin(), r->host_ids.end());
     std::vector<int> cidx(n0);
-    for (int k = 0; k < n0; ++k)
+    std::vector<unsigned long long> key0(n0);  // current voxel (device offset) in compact order
+    for (int k = 0; k < n0; ++k) {
         cidx[k] = (int)(std::lower_bound(r->host_ids.begin(), r->host_ids.end(), d.agent_order[k]) - r->host_ids.begin());
+        key0[cidx[k]] = (unsigned long long)d.agent_off[k];
+    }
     // the CSR may now need a segment per agent: new arrays, adopted once they are filled
     int *d_cidx = nullptr;
     int rc;
     const size_t tab = (size_t)ns * n0 * sizeof(double);
     if ((rc = hip_check(g, hipMalloc((void **)&r->ids, n0 * sizeof(int)), "agent ids")) ||
         (rc = hip_check(g, hipMalloc((void **)&r->alive, n0), "agent flags")) ||
+        (rc = hip_check(g, hipMalloc((void **)&r->key, n0 * sizeof(unsigned long long)), "agent voxels")) ||
         (rc = hip_check(g, hipMalloc((void **)&r->tab, 2 * tab), "agent tables")) ||
-        (rc = hip_check(g, hipMalloc((void **)&r->kin, n0 * sizeof(unsigned)), "agent keys")) ||
-        (rc = hip_check(g, hipMalloc((void **)&r->kout, n0 * sizeof(unsigned)), "agent keys")) ||
+        (rc = hip_check(g, hipMalloc((void **)&r->kin, n0 * sizeof(unsigned long long)), "agent keys")) ||
+        (rc = hip_check(g, hipMalloc((void **)&r->kout, n0 * sizeof(unsigned long long)), "agent keys")) ||
         (rc = hip_check(g, hipMalloc((void **)&r->vin, n0 * sizeof(int)), "agent order")) ||
         (rc = hip_check(g, hipMalloc((void **)&r->vout, n0 * sizeof(int)), "agent order")) ||
         (rc = hip_check(g, hipMalloc((void **)&r->head, n0 * sizeof(int)), "agent heads")) ||
@@ -202,7 +217,7 @@ int remap_setup(Group *g, Domain &d)
     {
         size_t b1 = 0, b2 = 0;
         if ((rc = hip_check(g, hipcub::DeviceRadixSort::SortPairs(nullptr, b1, r->kin, r->kout, r->vin, r->vout, n0, 0,
-                                                                   32, g->stream), "agent sort size")) ||
+                                                                   64, g->stream), "agent sort size")) ||
             (rc = hip_check(g, hipcub::DeviceScan::ExclusiveSum(nullptr, b2, r->head, r->sid, n0, g->stream),
                             "agent scan size")))
             goto fail;
@@ -213,6 +228,8 @@ int remap_setup(Group *g, Domain &d)
     if ((rc = hip_check(g, hipMemcpyAsync(r->ids, r->host_ids.data(), n0 * sizeof(int), hipMemcpyHostToDevice,
                                           g->stream), "agent ids")) ||
         (rc = hip_check(g, hipMemsetAsync(r->alive, 1, n0, g->stream), "agent flags")) ||
+        (rc = hip_check(g, hipMemcpyAsync(r->key, key0.data(), n0 * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                                          g->stream), "agent voxels")) ||
         (rc = hip_check(g, hipMemcpyAsync(d_cidx, cidx.data(), n0 * sizeof(int), hipMemcpyHostToDevice, g->stream),
                         "agent ranks")))
         goto fail;
@@ -231,15 +248,14 @@ fail:
 
 }  // namespace
 
-int domain_update_agent_positions(Group *g, int i, int n, const double *pos, const double *bbox_min, int on_device)
+int domain_update_agent_positions(Group *g, int i, int n, const double *pos, const double *bbox_min, int on_device,
+                                  bool keep_outside)
 {
     if (i < 0 || (size_t)i >= g->doms.size() || n < 0 || (n > 0 && !pos))
         return set_error(g, BFVMB_ERR_ARG, "update_agent_positions: bad argument");
     Domain &d = g->doms[i];
     if (n != d.agents_given)
         return set_error(g, BFVMB_ERR_ARG, "update_agent_positions: n differs from the agent list that was set");
-    if (d.nvox >= 0xffffffffull)
-        return set_error(g, BFVMB_ERR_ARG, "update_agent_positions: grid too large");
     int rc = hip_check(g, hipStreamSynchronize(g->stream), "agents sync");
     if (rc)
         return rc;
@@ -270,9 +286,10 @@ int domain_update_agent_positions(Group *g, int i, int n, const double *pos, con
             return rc;
         src = r->pos;
     }
-    const unsigned dead = (unsigned)d.nvox;
+    const unsigned long long dead = (unsigned long long)d.nvox;
     const dim3 grid((n0 + 255) / 256), block(256);
-    hipLaunchKernelGGL(k_agent_keys, grid, block, 0, g->stream, src, r->ids, r->alive, n0, m, r->kin, r->vin);
+    hipLaunchKernelGGL(k_agent_keys, grid, block, 0, g->stream, src, r->ids, r->alive, r->key, n0, m,
+                       keep_outside ? 1 : 0, r->kin, r->vin);
     size_t bytes = r->tmp_bytes;
     if ((rc = hip_check(g, hipGetLastError(), "agent keys")) ||
         (rc = hip_check(g, hipcub::DeviceRadixSort::SortPairs(r->tmp, bytes, r->kin, r->kout, r->vin, r->vout, n0, 0,
@@ -315,6 +332,7 @@ int domain_update_agent_positions(Group *g, int i, int n, const double *pos, con
         (void)hipFree(d.seg_off);
         d.seg_start = r->seg_start_new;
         d.seg_off = r->seg_off_new;
+        d.seg_cap = (size_t)n0;
         r->seg_start_new = nullptr;
         r->seg_off_new = nullptr;
     }
@@ -355,4 +373,16 @@ extern "C" int bfvmb_batch_update_agent_positions(bfvmb_batch *b, int i, int n, 
                                                   const double *bbox_min, int pos_on_device)
 {
     return b ? bfvmb::domain_update_agent_positions(b, i, n, pos, bbox_min, pos_on_device) : BFVMB_ERR_ARG;
+}
+
+extern "C" int bfvmb_assign_agent_positions(bfvmb_ctx *ctx, int n, const double *pos, const double *bbox_min,
+                                            int pos_on_device)
+{
+    return ctx ? bfvmb::domain_update_agent_positions(ctx, 0, n, pos, bbox_min, pos_on_device, true) : BFVMB_ERR_ARG;
+}
+
+extern "C" int bfvmb_batch_assign_agent_positions(bfvmb_batch *b, int i, int n, const double *pos,
+                                                  const double *bbox_min, int pos_on_device)
+{
+    return b ? bfvmb::domain_update_agent_positions(b, i, n, pos, bbox_min, pos_on_device, true) : BFVMB_ERR_ARG;
 }

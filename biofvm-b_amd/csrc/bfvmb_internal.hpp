@@ -100,8 +100,10 @@ struct Domain {
     int *seg_start = nullptr;
     long long *seg_off = nullptr;
     double *t1 = nullptr, *t2 = nullptr;
+    size_t seg_cap = 0, act_cap = 0;  // segments / agents the device arrays have room for
     std::vector<int> agent_order;  // sorted position -> index in the caller's agent list (after
                                    // moves: current once domain_agent_order has run)
+    std::vector<long long> agent_off;  // device offset of each sorted agent's voxel, as set
     int agents_given = 0;          // length of the caller's agent list
     struct AgentRemap *remap = nullptr;  // device regrouping state of moving agents (agents.hip)
     double *bulk = nullptr;        // bulk sources/sinks: T*S then U+S, each [nsub][nz][ny][nx]
@@ -176,7 +178,10 @@ int domain_set_agents(Group *g, int i, int n, const int *voxel, const double *te
 int domain_set_bulk(Group *g, int i, const double *supply, const double *target, const double *uptake);
 // moving agents (agents.hip): new positions of the agents that were set, voxels and
 // regrouping on the device; free_remap releases that state
-int domain_update_agent_positions(Group *g, int i, int n, const double *pos, const double *bbox_min, int on_device);
+// keep_outside: assign_position's rule (a position outside the box is rejected: the agent keeps
+// its voxel and stays active) instead of update_voxel_index's (the agent is deactivated)
+int domain_update_agent_positions(Group *g, int i, int n, const double *pos, const double *bbox_min, int on_device,
+                                  bool keep_outside = false);
 int domain_agent_order(Group *g, Domain &d);  // brings Domain::agent_order up to date
 // densities at listed voxels (field.hip); on_device: voxel and out are device pointers
 int domain_densities_at(Group *g, int i, int n, const int *voxel, double *out, int on_device);

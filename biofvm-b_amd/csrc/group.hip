@@ -124,6 +124,7 @@ void free_agents(Domain &d)
     d.seg_off = nullptr;
     d.t1 = d.t2 = nullptr;
     d.nseg = d.nact = 0;
+    d.seg_cap = d.act_cap = 0;
 }
 
 int sync_descriptors(Group *g)
@@ -505,24 +506,51 @@ int domain_set_agents(Group *g, int i, int n, const int *voxel, const double *te
     int rc = hip_check(g, hipStreamSynchronize(g->stream), "agents sync");
     if (rc)
         return rc;
-    free_agents(d);
+    free_remap(d);  // the regrouping state belongs to the previous list
+    const size_t nseg = seg_off.size();
+    // device arrays are reused while they have room (a list of the same size every call
+    // allocates nothing); the descriptor carries the counts, t1/t2 are strided by nact
+    if (nseg > d.seg_cap || (size_t)nact > d.act_cap) {
+        free_agents(d);
+        if (nact > 0 &&
+            ((rc = hip_check(g, hipMalloc((void **)&d.seg_start, (nseg + 1) * sizeof(int)), "agent segments")) ||
+             (rc = hip_check(g, hipMalloc((void **)&d.seg_off, nseg * sizeof(long long)), "agent voxels")) ||
+             (rc = hip_check(g, hipMalloc((void **)&d.t1, t1.size() * sizeof(double)), "agent temp1")) ||
+             (rc = hip_check(g, hipMalloc((void **)&d.t2, t2.size() * sizeof(double)), "agent temp2")))) {
+            free_agents(d);
+            d.agents_given = 0;
+            d.agent_order.clear();
+            g->desc_dirty = true;
+            g->epoch++;
+            return rc;
+        }
+        d.seg_cap = nact > 0 ? nseg : 0;
+        d.act_cap = (size_t)nact;
+    }
     d.agent_order = order;
     d.agents_given = n;
+    d.agent_off.resize(nact);
+    for (int k = 0; k < nact; ++k)
+        d.agent_off[k] = off[order[k]];
     g->desc_dirty = true;
     g->epoch++;
+    d.nseg = (int)nseg;
+    d.nact = nact;
     if (nact == 0)
         return BFVMB_OK;
-    d.nseg = (int)seg_off.size();
-    d.nact = nact;
-    if ((rc = hip_check(g, hipMalloc((void **)&d.seg_start, seg_start.size() * sizeof(int)), "agent segments")) ||
-        (rc = hip_check(g, hipMalloc((void **)&d.seg_off, seg_off.size() * sizeof(long long)), "agent voxels")) ||
-        (rc = hip_check(g, hipMalloc((void **)&d.t1, t1.size() * sizeof(double)), "agent temp1")) ||
-        (rc = hip_check(g, hipMalloc((void **)&d.t2, t2.size() * sizeof(double)), "agent temp2")) ||
-        (rc = hip_check(g, hipMemcpy(d.seg_start, seg_start.data(), seg_start.size() * sizeof(int), hipMemcpyHostToDevice), "agent upload")) ||
-        (rc = hip_check(g, hipMemcpy(d.seg_off, seg_off.data(), seg_off.size() * sizeof(long long), hipMemcpyHostToDevice), "agent upload")) ||
-        (rc = hip_check(g, hipMemcpy(d.t1, t1.data(), t1.size() * sizeof(double), hipMemcpyHostToDevice), "agent upload")) ||
-        (rc = hip_check(g, hipMemcpy(d.t2, t2.data(), t2.size() * sizeof(double), hipMemcpyHostToDevice), "agent upload"))) {
-        free_agents(d);
+    if ((rc = hip_check(g, hipMemcpyAsync(d.seg_start, seg_start.data(), seg_start.size() * sizeof(int),
+                                          hipMemcpyHostToDevice, g->stream), "agent upload")) ||
+        (rc = hip_check(g, hipMemcpyAsync(d.seg_off, seg_off.data(), nseg * sizeof(long long), hipMemcpyHostToDevice,
+                                          g->stream), "agent upload")) ||
+        (rc = hip_check(g, hipMemcpyAsync(d.t1, t1.data(), t1.size() * sizeof(double), hipMemcpyHostToDevice, g->stream),
+                        "agent upload")) ||
+        (rc = hip_check(g, hipMemcpyAsync(d.t2, t2.data(), t2.size() * sizeof(double), hipMemcpyHostToDevice, g->stream),
+                        "agent upload")) ||
+        (rc = hip_check(g, hipStreamSynchronize(g->stream), "agent upload"))) {
+        (void)hipStreamSynchronize(g->stream);
+        free_agents(d);  // no descriptor may keep pointing at a half-written list
+        d.agents_given = 0;
+        d.agent_order.clear();
         return rc;
     }
     return BFVMB_OK;

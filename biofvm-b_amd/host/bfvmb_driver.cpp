@@ -11,11 +11,21 @@
 //
 // usage: bfvmb_driver <case.bin> <out.bin> [fp64|fp32] [--fused] [--time] [--mat out.mat]
 //                     [--bulk rates.bin] [--sample samples.bin] [--dnodes nodes.bin]
+//                     [--moves moves.bin moves_out.bin]
+//        bfvmb_driver --bench-dropin <n> <nagents> <steps> [fp64|fp32]
+// --bench-dropin: the drop-in workload (n^3 voxels x 4 substrates, agents in a centred sphere
+// of radius 3 mm * n/400, lognormal(0, 0.5) fields; drawn with std::mt19937_64, for timing)
+// through the host layer, timing `steps` steps called one at a time
+// (simulate_cell_sources_and_sinks + simulate_diffusion_decay per step, as the reference's
+// drivers call the slots) against one run(agents, dt, steps); prints one JSON line.
 // --sample: at every snapshot, before the field is written, the densities at the agents'
 // voxels through Microenvironment::nearest_density_vectors (read from the device field),
 // [nagents][nsub] doubles per snapshot.
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
+#include <random>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -51,8 +61,88 @@ void bulk_rate(biofvmb::Microenvironment *, int v, double *dst)
 }
 }  // namespace
 
+int bench_dropin(int n, int nagents, int steps, int precision)
+{
+    const double dx = 20.0, dt = 0.01;
+    const double D[4] = {1e5, 1e3, 2.5e3, 5e4}, decay[4] = {0.1, 0.01, 0.4, 1e-3};
+    biofvmb::Microenvironment M(precision);
+    M.resize_densities(4);
+    for (int d = 0; d < 4; ++d)
+        M.set_density(d, "substrate" + std::to_string(d), "dimensionless", D[d], decay[d]);
+    M.resize_space_uniform(0.0, n * dx, 0.0, n * dx, 0.0, n * dx, dx);
+    for (int d = 0; d < 4; ++d) {
+        M.set_substrate_dirichlet_value(d, d == 0 ? 38.0 : 0.0);
+        M.set_substrate_dirichlet_activation(d, d == 0);
+    }
+    std::mt19937_64 gen(2024);
+    std::lognormal_distribution<double> logn(0.0, 0.5);
+    double *rho = M.density_data();
+    for (size_t i = 0; i < (size_t)n * n * n * 4; ++i)
+        rho[i] = logn(gen);
+    std::normal_distribution<double> normal(0.0, 1.0);
+    std::uniform_real_distribution<double> unit(0.0, 1.0);
+    const double c = n * dx / 2.0, radius = 3000.0 * n / 400.0;
+    for (int a = 0; a < nagents; ++a) {
+        double v[3], r2 = 0.0;
+        for (double &x : v) {
+            x = normal(gen);
+            r2 += x * x;
+        }
+        const double r = radius * std::cbrt(unit(gen)) / std::sqrt(r2);
+        biofvmb::Basic_Agent *ag = biofvmb::create_basic_agent();
+        ag->register_microenvironment(&M);
+        ag->assign_position(c + v[0] * r, c + v[1] * r, c + v[2] * r);
+        ag->set_total_volume(4.0 / 3.0 * M_PI * 125.0);
+        for (int d = 0; d < 4; ++d) {
+            ag->secretion_rates[d] = 5.0 * unit(gen);
+            ag->uptake_rates[d] = 20.0 * unit(gen);
+            ag->saturation_densities[d] = 1.0;
+        }
+        ag->set_internal_uptake_constants(dt);
+    }
+    auto now = [] { return std::chrono::steady_clock::now(); };
+    auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+        return std::chrono::duration<double, std::milli>(b - a).count();
+    };
+    for (int k = 0; k < 3; ++k) {  // warm-up: uploads, agent list, coefficients
+        M.simulate_cell_sources_and_sinks(dt);
+        M.simulate_diffusion_decay(dt);
+    }
+    M.run(biofvmb::all_basic_agents, dt, 2);
+    bfvmb_sync(M.context());
+    double per_call = 1e30, fused = 1e30;
+    for (int rep = 0; rep < 2; ++rep) {
+        auto t0 = now();
+        for (int k = 0; k < steps; ++k) {
+            M.simulate_cell_sources_and_sinks(dt);
+            M.simulate_diffusion_decay(dt);
+        }
+        bfvmb_sync(M.context());
+        auto t1 = now();
+        M.run(biofvmb::all_basic_agents, dt, steps);
+        bfvmb_sync(M.context());
+        auto t2 = now();
+        per_call = std::min(per_call, ms(t0, t1) / steps);
+        fused = std::min(fused, ms(t1, t2) / steps);
+    }
+    std::printf("{\"workload\": \"host layer, %d^3 x 4, %d agents\", \"steps\": %d, \"precision\": \"%s\", "
+                "\"per_call_ms_per_step\": %.6f, \"run_ms_per_step\": %.6f, \"per_call_over_run\": %.6f}\n",
+                n, nagents, steps, precision == BFVMB_FP64 ? "fp64" : "fp32", per_call, fused, per_call / fused);
+    biofvmb::delete_all_basic_agents();
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
+    if (argc >= 5 && !std::strcmp(argv[1], "--bench-dropin")) {
+        try {
+            return bench_dropin(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]),
+                                argc > 5 && !std::strcmp(argv[5], "fp32") ? BFVMB_FP32 : BFVMB_FP64);
+        } catch (const biofvmb::Error &e) {
+            std::fprintf(stderr, "bfvmb_driver: %s (code %d)\n", e.what(), e.code);
+            return 4;
+        }
+    }
     if (argc < 3) {
         std::fprintf(stderr, "usage: %s case.bin out.bin [fp64|fp32] [--fused] [--time] [--mat out.mat]\n", argv[0]);
         return 2;
@@ -63,7 +153,13 @@ int main(int argc, char **argv)
     const char *bulk_path = nullptr;  // --bulk PATH: supply, target, uptake [nvox][nsub] doubles
     const char *sample_path = nullptr;  // --sample PATH: densities at the agents' voxels per snapshot
     const char *dnodes_path = nullptr;  // --dnodes PATH: n, voxel[n], values[n][nsub] Dirichlet nodes
+    const char *moves_path = nullptr, *moves_out = nullptr;  // --moves IN OUT (the reference driver's format)
     for (int i = 3; i < argc; ++i) {
+        if (!std::strcmp(argv[i], "--moves") && i + 2 < argc) {
+            moves_path = argv[++i];
+            moves_out = argv[++i];
+            continue;
+        }
         if (!std::strcmp(argv[i], "--dnodes") && i + 1 < argc) {
             dnodes_path = argv[++i];
             continue;
@@ -205,15 +301,65 @@ int main(int argc, char **argv)
             if (fused)
                 M.update_rates();  // constant rates: run() applies them every step
         }
+        struct Move {
+            int step, route;
+            std::vector<double> pos;
+        };
+        std::vector<Move> moves;
+        if (moves_path) {
+            FILE *q = std::fopen(moves_path, "rb");
+            if (!q) {
+                std::perror(moves_path);
+                return 2;
+            }
+            const int nm = one<int>(q);
+            for (int k = 0; k < nm; ++k) {
+                Move mv;
+                mv.step = one<int>(q);
+                mv.route = one<int>(q);
+                mv.pos.resize(3 * (size_t)nagents);
+                get(q, mv.pos.data(), mv.pos.size());
+                moves.push_back(mv);
+            }
+            std::fclose(q);
+        }
+        FILE *mout = moves.empty() ? nullptr : std::fopen(moves_out, "wb");
+        auto move_agents = [&](int s) {  // before the cell update of step s, as the reference driver
+            for (const Move &mv : moves) {
+                if (mv.step != s)
+                    continue;
+                for (int a = 0; a < nagents; ++a) {
+                    biofvmb::Basic_Agent *ag = biofvmb::all_basic_agents[a];
+                    const double *p = &mv.pos[3 * (size_t)a];
+                    if (mv.route == 0) {
+                        ag->assign_position(p[0], p[1], p[2]);
+                    } else {
+                        ag->position = {p[0], p[1], p[2]};
+                        ag->update_voxel_index();
+                    }
+                }
+                for (int a = 0; a < nagents; ++a) {
+                    const int v = biofvmb::all_basic_agents[a]->get_current_voxel_index();
+                    std::fwrite(&v, sizeof(int), 1, mout);
+                }
+                for (int a = 0; a < nagents; ++a) {
+                    const int act = biofvmb::all_basic_agents[a]->is_active() ? 1 : 0;
+                    std::fwrite(&act, sizeof(int), 1, mout);
+                }
+            }
+        };
         int next = 0, done = 0;
         double ms = 0.0;
         for (int s = 1; s <= nsteps; ++s) {
             const bool snap = next < nsnap && snaps[next] == s;
-            if (fused && !snap && s < nsteps)
-                continue;  // batch steps up to the next snapshot into one call
+            const bool move_next = !moves.empty() && std::any_of(moves.begin(), moves.end(),
+                                                                 [&](const Move &m) { return m.step == s + 1; });
+            if (fused && !snap && s < nsteps && !move_next)
+                continue;  // batch steps up to the next snapshot (or move) into one call
             const int k = s - done;
             auto ta = std::chrono::steady_clock::now();
             if (fused) {
+                move_agents(done + 1);
                 std::vector<biofvmb::Basic_Agent *> none;
                 if (with_agents || bulk_path)
                     M.run(with_agents ? biofvmb::all_basic_agents : none, dt, k);
@@ -222,6 +368,7 @@ int main(int argc, char **argv)
                         M.simulate_diffusion_decay(dt);
             } else {
                 for (int i = 0; i < k; ++i) {
+                    move_agents(done + 1 + i);
                     if (bulk_path)
                         M.simulate_bulk_sources_and_sinks(dt);
                     if (with_agents)
@@ -249,6 +396,8 @@ int main(int argc, char **argv)
             }
         }
         std::fclose(out);
+        if (mout)
+            std::fclose(mout);
         if (mat_path)
             M.write_to_matlab(mat_path);
         if (timing)

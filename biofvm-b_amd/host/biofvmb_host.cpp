@@ -13,6 +13,7 @@ std::vector<Basic_Agent *> all_basic_agents;
 namespace {
 Microenvironment *g_default_micro = nullptr;  // default_microenvironment of the reference
 unsigned long long g_agent_epoch = 1;          // bumped by every agent mutation
+long long g_volume_pending = 0;                // agents whose volume changed since their constants
 }  // namespace
 
 // ------------------------------------------------------------------------- Basic_Agent
@@ -50,9 +51,23 @@ bool Basic_Agent::assign_position(double x, double y, double z)
     return true;
 }
 
+void Basic_Agent::update_voxel_index()
+{
+    const int v = micro_ ? micro_->nearest_voxel_index(position[0], position[1], position[2]) : -1;
+    if (v < 0) {
+        voxel_ = -1;
+        active_ = false;
+    } else {
+        voxel_ = v;
+    }
+    ++g_agent_epoch;
+}
+
 void Basic_Agent::set_total_volume(double v)
 {
     volume_ = v;
+    if (!volume_is_changed_)
+        ++g_volume_pending;
     volume_is_changed_ = true;
     ++g_agent_epoch;
 }
@@ -78,6 +93,8 @@ void Basic_Agent::set_internal_uptake_constants(double dt)
         temp1_[d] = t1;
         temp2_[d] = std::fma(k, uptake_rates[d], std::fma(k, secretion_rates[d], 1.0));
     }
+    if (volume_is_changed_)
+        --g_volume_pending;
     volume_is_changed_ = false;
     ++g_agent_epoch;
 }
@@ -92,8 +109,11 @@ Basic_Agent *create_basic_agent()
 
 void delete_all_basic_agents()
 {
-    for (Basic_Agent *a : all_basic_agents)
+    for (Basic_Agent *a : all_basic_agents) {
+        if (a->volume_is_changed_)
+            --g_volume_pending;
         delete a;
+    }
     all_basic_agents.clear();
     ++g_agent_epoch;
 }
@@ -364,10 +384,12 @@ void Microenvironment::simulate_diffusion_decay(double dt)
 
 void Microenvironment::upload_agents(std::vector<Basic_Agent *> &agents, double dt)
 {
-    // an agent whose volume changed gets new constants first (BioFVM_basic_agent.cpp:285-289)
-    for (Basic_Agent *a : agents)
-        if (a->active_ && a->volume_is_changed_)
-            a->set_internal_uptake_constants(dt);
+    // an agent whose volume changed gets new constants first (BioFVM_basic_agent.cpp:285-289);
+    // the list is walked only while some agent has such a change pending
+    if (g_volume_pending > 0)
+        for (Basic_Agent *a : agents)
+            if (a->active_ && a->volume_is_changed_)
+                a->set_internal_uptake_constants(dt);
     if (agents_uploaded_ && agents_key_ == &agents && agents_n_ == agents.size() && agents_epoch_ == g_agent_epoch)
         return;
     const int ns = number_of_densities();

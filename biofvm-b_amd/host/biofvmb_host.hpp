@@ -37,6 +37,9 @@ public:
     Basic_Agent();
     bool assign_position(double x, double y, double z);  // false outside the domain
     bool assign_position(const std::vector<double> &p) { return assign_position(p[0], p[1], p[2]); }
+    // BioFVM_basic_agent.cpp:137-153: the voxel of `position` as it stands; a position outside
+    // the domain deactivates the agent for good (voxel -1)
+    void update_voxel_index();
     void set_total_volume(double v);
     double get_total_volume() const { return volume_; }
     void set_internal_uptake_constants(double dt);  // BioFVM_basic_agent.cpp:155-176
@@ -55,6 +58,7 @@ public:
 
 private:
     friend class Microenvironment;
+    friend void delete_all_basic_agents();
     Microenvironment *micro_ = nullptr;
     double volume_ = 1.0;
     bool volume_is_changed_ = false;

@@ -120,10 +120,16 @@ int bfvmb_densities_at(bfvmb_ctx *ctx, int n, const int *voxel, double *out, int
  * BioFVM_parallel.cpp:418-522 at world size 1; the same arithmetic as bfvmb_voxel_indices).
  * The mapping, the stable regrouping by voxel and the reordering of temp1/temp2 run on the
  * device; the rates are kept.  An agent whose position lies outside the bounding box is
- * deactivated, as the reference does, and stays inactive until the agents are set again.
+ * deactivated, as update_voxel_index does, and stays inactive until the agents are set again.
  * pos_on_device != 0: pos is a device pointer, read in stream order.  Blocks until the new
  * grouping is in place (one small read-back). */
 int bfvmb_update_agent_positions(bfvmb_ctx *ctx, int n, const double *pos, const double *bbox_min,
+                                 int pos_on_device);
+
+/* The same move by Basic_Agent::assign_position (BioFVM_basic_agent.cpp:112-128): a position
+ * outside the bounding box is rejected, so that agent keeps its voxel and stays active; inside
+ * the box the voxel is update_voxel_index's.  Inactive agents stay inactive. */
+int bfvmb_assign_agent_positions(bfvmb_ctx *ctx, int n, const double *pos, const double *bbox_min,
                                  int pos_on_device);
 
 /* Microenvironment::simulate_cell_sources_and_sinks (BioFVM_microenvironment.cpp:1155-1169). */
@@ -263,8 +269,10 @@ int bfvmb_batch_upload_density(bfvmb_batch *b, int i, const double *rho_voxel_ma
 int bfvmb_batch_download_density(bfvmb_batch *b, int i, double *rho_voxel_major);
 int bfvmb_batch_set_agents(bfvmb_batch *b, int i, int n, const int *voxel, const double *temp1,
                            const double *temp2, const uint8_t *active);
-/* bfvmb_update_agent_positions for member i. */
+/* bfvmb_update_agent_positions / bfvmb_assign_agent_positions for member i. */
 int bfvmb_batch_update_agent_positions(bfvmb_batch *b, int i, int n, const double *pos,
+                                       const double *bbox_min, int pos_on_device);
+int bfvmb_batch_assign_agent_positions(bfvmb_batch *b, int i, int n, const double *pos,
                                        const double *bbox_min, int pos_on_device);
 /* bfvmb_set_bulk_rates for member i. */
 int bfvmb_batch_set_bulk_rates(bfvmb_batch *b, int i, const double *supply, const double *target,

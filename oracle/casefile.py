@@ -44,6 +44,8 @@ class Case:
     x0: float = 0.0
     bulk: object = None  # (supply, target, uptake), each [nx, ny, nz, nsub]; bulk update per step
     dnodes: object = None  # (voxel[n], values[n][nsub]): per-voxel Dirichlet values, in list order
+    moves: object = None  # [(step, route, positions [n][3])]: agents moved before that step's cell
+                          # update by assign_position (route 0) or position + update_voxel_index (1)
 
     @property
     def nagents(self) -> int:
@@ -67,6 +69,14 @@ class Case:
             np.array([len(voxel)], dtype=np.int32).tofile(f)
             np.ascontiguousarray(voxel, dtype=np.int32).tofile(f)
             np.ascontiguousarray(values, dtype=np.float64).tofile(f)
+
+    def write_moves(self, path: str) -> None:
+        """Agent moves as the drivers' --moves file: nmoves, then per move step, route, positions."""
+        with open(path, "wb") as f:
+            np.array([len(self.moves)], dtype=np.int32).tofile(f)
+            for step, route, pos in self.moves:
+                np.array([step, route], dtype=np.int32).tofile(f)
+                np.ascontiguousarray(pos, dtype=np.float64).reshape(self.nagents, 3).tofile(f)
 
     def write(self, path: str) -> None:
         n = self.nagents
@@ -135,6 +145,10 @@ def run_reference(case: Case, workdir: str, solver: str = "lod", granularity: in
         dpath = os.path.join(workdir, "dnodes.bin")
         case.write_dnodes(dpath)
         args += ["--dnodes", dpath]
+    if case.moves is not None:
+        mpath, mopath = os.path.join(workdir, "moves.bin"), os.path.join(workdir, "moves_out.bin")
+        case.write_moves(mpath)
+        args += ["--moves", mpath, mopath]
     proc = subprocess.run(args, env=env, capture_output=True, text=True, check=True)
     out = read_output(opath, case)
     out["stdout"] = proc.stdout
@@ -143,6 +157,9 @@ def run_reference(case: Case, workdir: str, solver: str = "lod", granularity: in
             out["mat"] = f.read()
     if grad:
         out["grad"] = np.fromfile(gpath, dtype=np.float64).reshape(case.nx, case.ny, case.nz, case.nsub, 3)
+    if case.moves is not None:
+        m = np.fromfile(mopath, dtype=np.int32).reshape(len(case.moves), 2, case.nagents)
+        out["move_voxel"], out["move_active"] = m[:, 0], m[:, 1]
     return out
 
 

@@ -139,6 +139,46 @@ def update_voxel_indices(case, pos, active):
     return voxel, act
 
 
+def assign_positions(case, pos, voxel, active):
+    """Basic_Agent::assign_position (BioFVM_basic_agent.cpp:112-128) for every agent at new
+    positions ``pos``: a position outside the bounding box is rejected (the agent keeps its
+    voxel and its activity); otherwise update_voxel_index's nearest voxel.  The activity flag
+    is never changed.  Returns (voxel, active)."""
+    bbox, pbbox = _d(case.bbox)
+    P = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    vox = np.array(voxel, dtype=np.int32, copy=True).reshape(-1)
+    for a in range(P.shape[0]):
+        p, pp = _d(P[a])
+        v = lib().orc_voxel_index(pp, pbbox, case.dx, case.dx, case.dx, case.nx, case.ny, case.nz)
+        if v >= 0:
+            vox[a] = v
+    return vox, np.array(active, dtype=np.uint8, copy=True).reshape(-1)
+
+
+def run_moves(case):
+    """The case's snapshots with its agent moves (case.moves) applied before the cell update
+    of their steps, as the reference driver applies them; also the (voxel, active) after each
+    move.  Voxels of agents deactivated by update_voxel_index read -1, as the reference's do."""
+    voxel, t1, t2 = agent_tables(case)
+    active = np.ones(case.nagents, np.uint8)
+    rho = np.array(case.init, dtype=np.float64, copy=True)
+    snaps, after = {}, []
+    moves = {step: (route, pos) for step, route, pos in case.moves}
+    for s in range(1, max(case.snaps) + 1):
+        if s in moves:
+            route, pos = moves[s]
+            if route == 0:
+                voxel, active = assign_positions(case, pos, voxel, active)
+            else:
+                voxel, active = update_voxel_indices(case, pos, active)
+            after.append((voxel.copy(), active.copy()))
+        sources_sinks(rho, np.maximum(voxel, 0), active, t1, t2)
+        rho = run(case, nsteps=1, rho=rho, with_agents=False)
+        if s in case.snaps:
+            snaps[s] = rho.copy()
+    return snaps, after
+
+
 def agent_tables(case):
     voxel = voxel_indices(case)
     dV = (case.dx * case.dx) * case.dx

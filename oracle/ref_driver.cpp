@@ -14,6 +14,7 @@
 //
 // usage: ref_driver <case.bin> <out.bin> [lod|avx] [granularity] [--time] [--mat out.mat]
 //                   [--bulk rates.bin] [--grad grad.bin] [--dnodes nodes.bin] [--serial-agents]
+//                   [--moves moves.bin moves_out.bin]
 //        ref_driver_hip <case.bin> <out.bin> [hip|hipres] ...   (BFVMB_PLUGIN build: the
 //        GPU solver in the diffusion_decay_solver slot; hipres keeps the field resident)
 #include <mpi.h>
@@ -41,6 +42,7 @@ namespace {
 struct AgentPeek : Basic_Agent {
     static std::vector<double> Basic_Agent::*temp1() { return &AgentPeek::cell_source_sink_solver_temp1; }
     static std::vector<double> Basic_Agent::*temp2() { return &AgentPeek::cell_source_sink_solver_temp2; }
+    static bool Basic_Agent::*active() { return &AgentPeek::is_active; }
 };
 
 struct Reader {
@@ -95,6 +97,16 @@ int main(int argc, char **argv)
     for (int i = 3; i < argc; ++i)
         if (std::string(argv[i]) == "--serial-agents")
             serial_agents = true;
+    // --moves IN OUT: agent moves, IN = nmoves, then per move: step, route, positions [n][3];
+    // before the cell update of that step every agent moves by assign_position (route 0) or by
+    // setting its position and calling update_voxel_index (route 1); OUT gets, per move, the
+    // agents' voxels [n] (int) and active flags [n] (int) after it
+    std::string moves_path, moves_out;
+    for (int i = 3; i + 2 < argc; ++i)
+        if (std::string(argv[i]) == "--moves") {
+            moves_path = argv[i + 1];
+            moves_out = argv[i + 2];
+        }
     std::string dnodes_path;  // --dnodes PATH: n, voxel[n], values[n][nsub]: per-voxel Dirichlet
                               // values set through the four public node setters in turn
     for (int i = 3; i + 1 < argc; ++i) {
@@ -238,6 +250,26 @@ int main(int argc, char **argv)
         }
     }
 
+    struct Move {
+        int step, route;
+        std::vector<double> pos;
+    };
+    std::vector<Move> moves;
+    if (!moves_path.empty()) {
+        Reader q{fopen(moves_path.c_str(), "rb")};
+        if (!q.f) { perror(moves_path.c_str()); return 2; }
+        const int nm = q.one<int>();
+        for (int k = 0; k < nm; ++k) {
+            Move mv;
+            mv.step = q.one<int>();
+            mv.route = q.one<int>();
+            mv.pos.resize(3 * (size_t)nagents);
+            q.get(mv.pos.data(), mv.pos.size());
+            moves.push_back(mv);
+        }
+        fclose(q.f);
+    }
+    FILE *mout = moves.empty() ? nullptr : fopen(moves_out.c_str(), "wb");
     FILE *out = fopen(argv[2], "wb");
     if (!out) { perror(argv[2]); return 2; }
     fwrite("BFVMOUT1", 1, 8, out);
@@ -251,6 +283,30 @@ int main(int argc, char **argv)
     double total_ms = 0.0;
     for (int s = 1; s <= nsteps; ++s) {
         auto t_a = std::chrono::steady_clock::now();
+        for (const Move &mv : moves) {
+            if (mv.step != s)
+                continue;
+            for (int a = 0; a < nagents; ++a) {
+                Basic_Agent *ag = all_basic_agents[a];
+                const double *p = &mv.pos[3 * (size_t)a];
+                if (mv.route == 0) {
+                    ag->assign_position(p[0], p[1], p[2], rank, dims);
+                } else {
+                    ag->position[0] = p[0];
+                    ag->position[1] = p[1];
+                    ag->position[2] = p[2];
+                    ag->update_voxel_index(rank, dims);
+                }
+            }
+            for (int a = 0; a < nagents; ++a) {
+                const int v = all_basic_agents[a]->get_current_voxel_index();
+                fwrite(&v, sizeof(int), 1, mout);
+            }
+            for (int a = 0; a < nagents; ++a) {
+                const int act = all_basic_agents[a]->*AgentPeek::active() ? 1 : 0;
+                fwrite(&act, sizeof(int), 1, mout);
+            }
+        }
         if (!bulk_path.empty()) {
 #ifdef BFVMB_PLUGIN
             if (hip)
@@ -291,6 +347,8 @@ int main(int argc, char **argv)
         }
     }
     fclose(out);
+    if (mout)
+        fclose(mout);
     if (!grad_path.empty()) {
 #ifdef BFVMB_PLUGIN
         if (hip)

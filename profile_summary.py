@@ -74,7 +74,20 @@ def traffic(fetch_db, write_db, out):
         for pre, ph in phase.items():
             if s.startswith(pre) and "double" in s:
                 res[ph] = fb + wb
-    json.dump({"bytes_per_launch": res, "calibration": cal}, open(out, "w"), indent=1)
+    # provenance: the sweep source these counters describe (bench.py reports the traffic only
+    # while biofvm-b_amd/csrc/lod.hip still has this hash) and the commit checked out
+    import hashlib
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.abspath(__file__))
+    sha = hashlib.sha256(open(os.path.join(root, "biofvm-b_amd", "csrc", "lod.hip"), "rb").read()).hexdigest()
+    try:
+        commit = subprocess.run(["git", "-C", root, "rev-parse", "--short=12", "HEAD"], capture_output=True,
+                                text=True, check=True).stdout.strip()
+    except (OSError, subprocess.CalledProcessError):
+        commit = None
+    json.dump({"bytes_per_launch": res, "calibration": cal, "lod_hip_sha256": sha, "commit": commit},
+              open(out, "w"), indent=1)
     print(f"calibration (true bytes / counter bytes of the 1 GiB copy): fetch {cal['fetch']:.3f}, write {cal['write']:.3f}")
     print("| kernel | fetched (GB/launch) | written (GB/launch) | total (GB) |")
     print("|---|---|---|---|")

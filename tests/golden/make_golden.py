@@ -12,6 +12,7 @@ scalar solver; that is recorded in the fixture.
     python tests/golden/make_golden.py --bulk   # bulk source/sink fixtures
     python tests/golden/make_golden.py --grad   # gradient fixtures (field + gradients)
     python tests/golden/make_golden.py --dnodes # per-voxel Dirichlet fixtures
+    python tests/golden/make_golden.py --moves  # agents moved by assign_position / update_voxel_index
     python tests/golden/make_golden.py --dropin # the drop-in workload at full size (400^3 x 4,
                                                 # 1M agents, 200 steps; ~50 GB of memory): sampled
                                                 # voxels and whole-field checksums at steps 1, 10, 200
@@ -132,6 +133,42 @@ def main_dnodes():
         print(f"{name}: {os.path.getsize(path) / 1e3:.0f} kB, {len(case.dnodes[0])} nodes, avx==lod {avx_equal}")
 
 
+def moves_case():
+    """Agents moved three times through both public routes (BioFVM_basic_agent.cpp:112-153):
+    assign_position (rejects outside positions), then position + update_voxel_index
+    (deactivates), then assign_position again (deactivated agents come back inside but stay
+    inactive; active ones sent outside keep their voxels); shared voxels throughout."""
+    c = cf.make_case(23, 19, 17, 2, seed=30, nagents=400, nsteps=7, snaps=[1, 3, 5, 7], shared_voxels=True,
+                     outside=4, dactive=[1, 0])
+    rng = np.random.default_rng(31)
+    ext = np.array([c.nx, c.ny, c.nz]) * c.dx
+    p1 = c.pos + rng.normal(0.0, 2.0 * c.dx, c.pos.shape)
+    p1[:30] = -5.0                                   # rejected: keep voxel, stay active
+    p1[30:50] = ext * np.array([0.5, 0.25, 0.75])    # piled into one voxel
+    p2 = p1 + rng.normal(0.0, 2.0 * c.dx, p1.shape)
+    p2[10:35] = ext * 1.2                            # deactivated
+    p3 = p2 + rng.normal(0.0, 1.0 * c.dx, p2.shape)
+    p3[10:20] = ext * 0.5                            # back inside: stay inactive
+    p3[100:110] = -1.0                               # rejected again
+    p3[200:215] = ext                                # upper corner, inside
+    c.moves = [(2, 0, p1), (4, 1, p2), (6, 0, p3)]
+    return c
+
+
+def main_moves():
+    case = moves_case()
+    with tempfile.TemporaryDirectory() as d:
+        out = cf.run_reference(case, d, solver="lod", granularity=1)
+    path = save("moves_23x19x17x2", case, out, avx_equal=True)
+    act = out["move_active"]
+    print(f"{path}: active after moves {act.sum(axis=1).tolist()} of {case.nagents}")
+
+
+def load_moves():
+    case, ref = load("moves_23x19x17x2")
+    return case, ref
+
+
 DROPIN_SNAPS = [1, 10, 200]
 DROPIN_SAMPLES = 4096
 
@@ -242,6 +279,11 @@ def save(name, case, out, avx_equal):
         arrays["grad"] = out["grad"]
     if case.dnodes is not None:
         arrays["dnodes_voxel"], arrays["dnodes_values"] = case.dnodes
+    if case.moves is not None:
+        arrays["moves_step"] = np.array([m[0] for m in case.moves], np.int32)
+        arrays["moves_route"] = np.array([m[1] for m in case.moves], np.int32)
+        arrays["moves_pos"] = np.stack([m[2] for m in case.moves])
+        arrays["move_voxel"], arrays["move_active"] = out["move_voxel"], out["move_active"]
     for s, f in out["fields"].items():
         arrays[f"field_{s}"] = f
     path = os.path.join(HERE, f"{name}.npz")
@@ -274,6 +316,9 @@ def load(name):
         ref["grad"] = z["grad"]
     if "dnodes_voxel" in z.files:
         case.dnodes = (z["dnodes_voxel"], z["dnodes_values"])
+    if "moves_step" in z.files:
+        case.moves = [(int(a), int(b), p) for a, b, p in zip(z["moves_step"], z["moves_route"], z["moves_pos"])]
+        ref["move_voxel"], ref["move_active"] = z["move_voxel"], z["move_active"]
     return case, ref
 
 
@@ -302,6 +347,8 @@ if __name__ == "__main__":
         main_grad()
     elif "--dnodes" in sys.argv[1:]:
         main_dnodes()
+    elif "--moves" in sys.argv[1:]:
+        main_moves()
     elif "--dropin" in sys.argv[1:]:
         main_dropin()
     else:

@@ -201,3 +201,128 @@ def test_last_segment_shared_interior_voxel(precision):
         rho = got.astype(np.float64)  # continue from the library's field (FP32: its rounding)
     assert 0 < int(active.sum()) < case.nagents
     s.close()
+
+
+# ---- both public routes of the reference, against its own run (tests/golden/moves_*.npz)
+def _fixture():
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    import make_golden as golden
+    return golden.load_moves()
+
+
+def _run_moves_capi(case, ref, precision, device_positions=False):
+    """The fixture's moves through the C ABI: route 0 = bfvmb_assign_agent_positions, route 1 =
+    bfvmb_update_agent_positions, before the step's cell update."""
+    from biofvmb import LODSolver
+    s = LODSolver(case.nx, case.ny, case.nz, case.nsub, case.dx, precision)
+    s.set_coefficients(case.D, case.decay, case.dt)
+    s.set_dirichlet(*cf.effective_dirichlet(case))
+    s.upload(case.init)
+    s.set_agent_rates(ref["voxel"], case.volume, case.S, case.U, case.T, case.dt)
+    moves = {st: (r, p) for st, r, p in case.moves}
+    out = {}
+    for step in range(1, max(case.snaps) + 1):
+        if step in moves:
+            route, pos = moves[step]
+            if device_positions:
+                import torch
+                pos = torch.tensor(pos, dtype=torch.float64, device="cuda:0").contiguous()
+                torch.cuda.synchronize()
+            (s.assign_agent_positions if route == 0 else s.update_agent_positions)(pos)
+        s.run(1)
+        if step in case.snaps:
+            out[step] = s.download()
+    s.close()
+    return out
+
+
+@pytest.mark.parametrize("device_positions", [False, True])
+def test_both_move_routes_match_reference_fp64(device_positions):
+    case, ref = _fixture()
+    print(f"[gpu] reference moves fixture, device positions {device_positions}", flush=True)
+    got = _run_moves_capi(case, ref, "fp64", device_positions)
+    for step, f in ref["fields"].items():
+        assert np.array_equal(got[step], f), f"step {step}: {(got[step] != f).sum()} differ"
+
+
+def test_both_move_routes_fp32_error():
+    case, ref = _fixture()
+    print("[gpu] reference moves fixture fp32", flush=True)
+    got = _run_moves_capi(case, ref, "fp32")
+    for step, f in ref["fields"].items():
+        r = np.abs(got[step] - f) / np.maximum(np.abs(f), 1e-300)
+        print(f"step {step}: fp32 max rel {r.max():.3e}")
+        assert r.max() <= 1e-5
+
+
+@pytest.mark.parametrize("mode", ["per_call", "fused"])
+def test_host_layer_moves_match_reference(mode):
+    """Basic_Agent::assign_position / update_voxel_index of the C++ host layer, in the
+    reference driver's order: voxels and activity after each move and the fields."""
+    import os
+    import subprocess
+    import tempfile
+    case, ref = _fixture()
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "biofvm-b_amd", "lib", "bfvmb_driver")
+    print(f"[gpu] host layer moves {mode}", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        cpath, opath, mpath, mo = (os.path.join(d, f) for f in ("case.bin", "out.bin", "moves.bin", "mo.bin"))
+        case.write(cpath)
+        case.write_moves(mpath)
+        args = [exe, cpath, opath, "fp64", "--moves", mpath, mo] + (["--fused"] if mode == "fused" else [])
+        subprocess.run(args, check=True, timeout=120)
+        out = cf.read_output(opath, case)
+        m = np.fromfile(mo, dtype=np.int32).reshape(len(case.moves), 2, case.nagents)
+    assert np.array_equal(m[:, 0], ref["move_voxel"]) and np.array_equal(m[:, 1], ref["move_active"])
+    for step, f in ref["fields"].items():
+        assert np.array_equal(out["fields"][step], f), f"step {step}"
+
+
+@pytest.mark.parametrize("mode", ["hip", "hipres"])
+def test_reference_driver_moves_with_gpu_slots(mode):
+    """The reference's own agents moved by its own routes, with the GPU in its slots: the plugin
+    re-uploads the agent list when voxels or activity changed."""
+    import os
+    import subprocess
+    import tempfile
+    case, ref = _fixture()
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "oracle", "_ref", "ref_driver_hip")
+    if not os.path.exists(exe):
+        pytest.skip("reference build with the plugin absent")
+    print(f"[gpu] reference driver moves {mode}", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        cpath, opath, mpath, mo = (os.path.join(d, f) for f in ("case.bin", "out.bin", "moves.bin", "mo.bin"))
+        case.write(cpath)
+        case.write_moves(mpath)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([exe, cpath, opath, mode, "1", "--moves", mpath, mo], check=True, timeout=120, env=env)
+        out = cf.read_output(opath, case)
+    for step, f in ref["fields"].items():
+        assert np.array_equal(out["fields"][step], f), f"step {step}"
+
+
+def test_batch_member_assign_moves():
+    from biofvmb import Batch
+    case, ref = _fixture()
+    other = cf.make_case(11, 9, 13, 2, seed=2, nagents=50, nsteps=1)
+    print("[gpu] batch member, both move routes", flush=True)
+    b = Batch([(other.nx, other.ny, other.nz, other.nsub), (case.nx, case.ny, case.nz, case.nsub)], case.dx, "fp64")
+    for i, c in enumerate((other, case)):
+        b.set_coefficients(i, c.D, c.decay, c.dt)
+        b.set_dirichlet(i, *cf.effective_dirichlet(c))
+        b.upload(i, c.init)
+        voxel, t1, t2 = orc.agent_tables(c)
+        b.set_agents(i, voxel, t1, t2)
+    moves = {st: (r, p) for st, r, p in case.moves}
+    for step in range(1, max(case.snaps) + 1):
+        if step in moves:
+            route, pos = moves[step]
+            (b.assign_agent_positions if route == 0 else b.update_agent_positions)(1, pos, case.nagents)
+        b.run(1)
+        if step in case.snaps:
+            assert np.array_equal(b.download(1), ref["fields"][step]), step
+    b.close()

@@ -296,3 +296,21 @@ def test_gradients_axis_of_length_one_is_zero():
     rho = np.random.default_rng(0).uniform(size=(1, 4, 1, 2))
     g = orc.gradients(rho, 20.0)
     assert np.all(g[..., 0] == 0.0) and np.all(g[..., 2] == 0.0) and np.any(g[..., 1] != 0.0)
+
+
+def test_agent_moves_match_reference():
+    """Both public routes that move an agent (assign_position, and position +
+    update_voxel_index; BioFVM_basic_agent.cpp:112-153), as the reference applied them: the
+    voxels and activity after every move, and the fields, bit for bit."""
+    case, ref = golden.load_moves()
+    snaps, after = orc.run_moves(case)
+    for k, (voxel, active) in enumerate(after):
+        assert np.array_equal(voxel, ref["move_voxel"][k]), f"move {k}: voxels"
+        assert np.array_equal(active, ref["move_active"][k].astype(np.uint8)), f"move {k}: activity"
+    for s, f in ref["fields"].items():
+        assert np.array_equal(snaps[s], f), f"step {s}"
+    # the routes differ on this case: swapping them changes the result
+    case.moves = [(st, 1 - r, p) for st, r, p in case.moves]
+    swapped, _ = orc.run_moves(case)
+    last = max(ref["fields"])
+    assert not np.array_equal(swapped[last], ref["fields"][last])
