@@ -76,6 +76,7 @@ SIGNATURES = {
     "bfvmb_batch_autotune": (_c.c_int, [_vp, _c.c_int, _ip]),
     "bfvmb_update_agent_positions": (_c.c_int, [_vp, _c.c_int, _vp, _dp, _c.c_int]),
     "bfvmb_densities_at": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _c.c_int]),
+    "bfvmb_densities_at_noalloc": (_c.c_int, [_vp, _c.c_int, _ip, _dp]),
     "bfvmb_upload_density_device": (_c.c_int, [_vp, _vp]),
     "bfvmb_download_density_device": (_c.c_int, [_vp, _vp]),
     "bfvmb_adjoint_backward_device": (_c.c_int, [_vp, _vp, _vp, _vp, _vp]),

@@ -181,6 +181,15 @@ class LODSolver:
                                                  ctypes.c_void_p(res.ctypes.data), 0), "densities_at")
         return res
 
+    def densities_at_noalloc(self, voxel):
+        """densities_at from host arrays through bfvmb_densities_at_noalloc: staged in the
+        library's static device scratch, no device allocation."""
+        v = np.ascontiguousarray(voxel, dtype=np.int32).reshape(-1)
+        res = np.empty((v.size, self.nsub), dtype=np.float64)
+        self._check(self._lib.bfvmb_densities_at_noalloc(self._h, v.size, L.iptr(v), L.dptr(res)),
+                    "densities_at_noalloc")
+        return res
+
     def update_agent_positions(self, pos, bbox_min=(0.0, 0.0, 0.0)) -> None:
         """New positions ([n][3], host array or CUDA float64 tensor) of the agents last set:
         voxels and the grouping by voxel are recomputed on the device; agents outside the

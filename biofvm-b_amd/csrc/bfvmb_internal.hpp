@@ -185,6 +185,10 @@ int domain_update_agent_positions(Group *g, int i, int n, const double *pos, con
 int domain_agent_order(Group *g, Domain &d);  // brings Domain::agent_order up to date
 // densities at listed voxels (field.hip); on_device: voxel and out are device pointers
 int domain_densities_at(Group *g, int i, int n, const int *voxel, double *out, int on_device);
+// the same from host arrays, staged in the library's static device scratch (no allocation);
+// scratch_setup finds that array's address on the group's device (group_init calls it)
+int domain_densities_at_noalloc(Group *g, int i, int n, const int *voxel, double *out);
+int scratch_setup(Group *g);
 int domain_upload_device(Group *g, int i, const double *d_rho);   // group.hip, stream-ordered
 int domain_download_device(Group *g, int i, double *d_rho);
 void free_remap(Domain &d);

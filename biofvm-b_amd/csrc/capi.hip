@@ -159,6 +159,11 @@ int bfvmb_densities_at(bfvmb_ctx *ctx, int n, const int *voxel, double *out, int
     return ctx ? domain_densities_at(ctx, 0, n, voxel, out, on_device) : BFVMB_ERR_ARG;
 }
 
+int bfvmb_densities_at_noalloc(bfvmb_ctx *ctx, int n, const int *voxel, double *out)
+{
+    return ctx ? domain_densities_at_noalloc(ctx, 0, n, voxel, out) : BFVMB_ERR_ARG;
+}
+
 int bfvmb_simulate_cell_sources_and_sinks(bfvmb_ctx *ctx)
 {
     if (!ctx)

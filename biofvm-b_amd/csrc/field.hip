@@ -1,5 +1,8 @@
 // field.hip -- agent source/sink kernel and the layout transposes between the reference's
 // voxel-major host field and the device's [substrate][z][y][x] field.
+#include <mutex>
+#include <string>
+
 #include "bfvmb_internal.hpp"
 
 namespace bfvmb {
@@ -203,6 +206,83 @@ int domain_densities_at(Group *g, int i, int n, const int *voxel, double *out, i
     (void)hipFree(tv);
     (void)hipFree(tout);
     return rc;
+}
+
+// ---- densities at listed voxels without allocating (bfvmb_densities_at_noalloc).  The voxel
+// list and the gathered values are staged in `scratch`, a file-scope device array that every
+// GPU the library runs on holds until the process ends.  Host code never passes the array's
+// name to a kernel or a copy: that is a host-side stand-in, not the device address.  group_init
+// looks the address up once per device (hipGetSymbolAddress, with the device current) and
+// every later use goes through that pointer.  All calls on one GPU share the array, so calls
+// that overlap on one GPU (two host threads, two streams) race; one call at a time is safe.
+constexpr size_t kScratchBytes = BFVMB_NOALLOC_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int scratch_setup(Group *g)
+{
+    if (g->device < 0 || g->device >= kMaxDevices)
+        return set_error(g, BFVMB_ERR_NODEV, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[g->device])
+        return BFVMB_OK;
+    void *p = nullptr;
+    int rc = hip_check(g, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "scratch address");
+    if (rc)
+        return rc;
+    if (!p)
+        return set_error(g, BFVMB_ERR_HIP, "scratch address: null");
+    g_scratch_ptr[g->device] = p;
+    return BFVMB_OK;
+}
+
+int domain_densities_at_noalloc(Group *g, int i, int n, const int *voxel, double *out)
+{
+    if (i < 0 || (size_t)i >= g->doms.size() || n < 0 || (n > 0 && (!voxel || !out)))
+        return set_error(g, BFVMB_ERR_ARG, "densities_at_noalloc: bad argument");
+    Domain &d = g->doms[i];
+    for (int k = 0; k < n; ++k)
+        if (voxel[k] < 0 || (size_t)voxel[k] >= d.nvox)
+            return set_error(g, BFVMB_ERR_ARG, "densities_at_noalloc: voxel index out of range");
+    const size_t vbytes = ((size_t)n * sizeof(int) + 255) / 256 * 256;
+    const size_t obytes = (size_t)n * d.nsub * sizeof(double);
+    if (vbytes + obytes > kScratchBytes)
+        return set_error(g, BFVMB_ERR_ARG,
+                         "densities_at_noalloc: " + std::to_string(n) + " voxels x " + std::to_string(d.nsub) +
+                             " substrates need " + std::to_string(vbytes + obytes) + " bytes of scratch, above its " +
+                             std::to_string(kScratchBytes) + " (use bfvmb_densities_at)");
+    if (n == 0)
+        return BFVMB_OK;
+    unsigned char *base;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        base = static_cast<unsigned char *>(g->device >= 0 && g->device < kMaxDevices ? g_scratch_ptr[g->device]
+                                                                                       : nullptr);
+    }
+    if (!base)
+        return set_error(g, BFVMB_ERR_STATE, "densities_at_noalloc: scratch not set up on this device");
+    int *tv = reinterpret_cast<int *>(base);
+    double *tout = reinterpret_cast<double *>(base + vbytes);
+    int rc = hip_check(g, hipMemcpyAsync(tv, voxel, (size_t)n * sizeof(int), hipMemcpyHostToDevice, g->stream),
+                       "voxels");
+    if (rc)
+        return rc;
+    const unsigned grid = (unsigned)(((long long)n * d.nsub + 255) / 256);
+    if (g->precision == BFVMB_FP64)
+        hipLaunchKernelGGL(k_densities_at<double>, dim3(grid), dim3(256), 0, g->stream,
+                           static_cast<const double *>(d.field[g->cur]), d.nx, d.ny, d.nz, d.nsub, tv, n, tout);
+    else
+        hipLaunchKernelGGL(k_densities_at<float>, dim3(grid), dim3(256), 0, g->stream,
+                           static_cast<const float *>(d.field[g->cur]), d.nx, d.ny, d.nz, d.nsub, tv, n, tout);
+    if ((rc = hip_check(g, hipGetLastError(), "densities_at_noalloc launch")) ||
+        (rc = hip_check(g, hipMemcpyAsync(out, tout, obytes, hipMemcpyDeviceToHost, g->stream), "densities")))
+        return rc;
+    return hip_check(g, hipStreamSynchronize(g->stream), "densities sync");
 }
 
 int launch_agents(Group *g)

@@ -201,7 +201,8 @@ int group_init(Group *g, int ndom, const int *nx, const int *ny, const int *nz, 
         (rc = hip_check(g, hipStreamCreate(&g->own_stream), "hipStreamCreate")))
         return rc;
     g->stream = g->own_stream;
-    if ((rc = hip_check(g, (hipError_t)sweep_lds_attributes(g->max_lds_per_block), "LDS attributes")))
+    if ((rc = hip_check(g, (hipError_t)sweep_lds_attributes(g->max_lds_per_block), "LDS attributes")) ||
+        (rc = scratch_setup(g)))
         return rc;
     g->doms.resize(ndom);
     for (int i = 0; i < ndom; ++i) {

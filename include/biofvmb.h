@@ -114,6 +114,17 @@ int bfvmb_voxel_indices(int nx, int ny, int nz, double dx, const double *bbox_mi
  * filled. */
 int bfvmb_densities_at(bfvmb_ctx *ctx, int n, const int *voxel, double *out, int on_device);
 
+/* bfvmb_densities_at from host arrays (voxel[n], out[n][nsub]) that allocates no device memory:
+ * the voxel list and the gathered values are staged in a static device array of
+ * BFVMB_NOALLOC_SCRATCH_BYTES that the library keeps on every GPU it runs on until the process
+ * ends.  No counterpart in the reference: it is for callers that cannot allocate (tight
+ * per-step sensing loops, code that must not call hipMalloc).  Requires
+ * 256*ceil(4n/256) + 8*n*nsub <= BFVMB_NOALLOC_SCRATCH_BYTES, else BFVMB_ERR_ARG.  All calls on
+ * one GPU share that array: calls that overlap on one GPU (other host threads, other streams)
+ * race.  Blocks until out is filled. */
+#define BFVMB_NOALLOC_SCRATCH_BYTES (16u << 20)
+int bfvmb_densities_at_noalloc(bfvmb_ctx *ctx, int n, const int *voxel, double *out);
+
 /* Moving agents: new positions ([n][3], micrometres; grid lower corner bbox_min, NULL = the
  * origin) of the n agents last given to bfvmb_set_agents / bfvmb_set_agent_rates, as
  * Basic_Agent::update_voxel_index computes their voxels (BioFVM_basic_agent.cpp:137-153 with
