@@ -259,6 +259,31 @@ def traffic_record(dom):
     return rec.get("bytes_per_launch", {}).get(dom), meta
 
 
+def copy_ceiling(torch, reps=20):
+    """Streaming ceilings of this GPU for a field-sized buffer (400^3 x 4 doubles, 2.048 GB):
+    device copy (read + write), read-only (sum) and write-only (fill), each timed with HIP
+    events over `reps` launches; GB/s counted as bytes moved."""
+    n = N ** 3 * NSUB
+    a = torch.rand(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    out = {}
+    for name, fn, bytes_moved in (("copy", lambda: b.copy_(a), 16 * n), ("read", lambda: a.sum(), 8 * n),
+                                  ("write", lambda: b.fill_(1.0), 8 * n)):
+        for _ in range(3):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        sec = e0.elapsed_time(e1) / 1e3 / reps
+        out[name] = {"GB_per_s": bytes_moved / sec / 1e9, "ms": sec * 1e3, "frac_of_peak": bytes_moved / sec / 1e9 / HBM_PEAK_GBS}
+        log(f"{name}: {bytes_moved / sec / 1e9:.1f} GB/s")
+    del a, b
+    return out
+
+
 def cpu_baseline(repeats=3):
     """The reference (oracle/_ref) on a bounded sample: 200^3 x 4 with 125k agents (1/8 of
     the domain, same densities), 16 steps, median of three runs (≈20 s of CPU work on 16
@@ -309,6 +334,8 @@ def main():
                     help="time only the batched workload and print its JSON object (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device copy (reference for HBM byte counters)")
+    ap.add_argument("--copy-ceiling", action="store_true",
+                    help="only time streaming copy / read / write of a field-sized buffer and print them")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the LOD path is built and measured on a single MI355X "
@@ -324,6 +351,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    if args.copy_ceiling:
+        print(json.dumps({"copy_ceiling": copy_ceiling(torch), "bytes": N ** 3 * NSUB * 8}), flush=True)
+        return
     if args.batched_only:
         print(json.dumps(bench_batched(torch, stream)), flush=True)
         return
