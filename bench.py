@@ -121,28 +121,36 @@ def bench_batched(torch, stream, steps=20, seed=3):
     return out
 
 
-def bench_c5(torch, stream, steps=10, nagents=10_000_000, seed=9):
-    """FP32 bulk path at 640^3 x 8 substrates (log-spaced D and decay, Dirichlet on
-    substrates 0-1, 10M agents in a centred 5 mm sphere), constant initial field."""
+def bench_c5(torch, stream, steps=10, nagents=10_000_000, seed=9, precisions=("fp32",)):
+    """The large workload, 640^3 x 8 substrates (log-spaced D and decay, Dirichlet on
+    substrates 0-1, 10M agents in a centred 5 mm sphere), initial field 1: FP32 storage (the
+    bulk path) and, when asked, FP64, with per-phase kernel times."""
     from biofvmb import LODSolver, workloads
     n, ns = workloads.LARGE["n"], workloads.LARGE["nsub"]
     Dm, lam, dvalue, dactive, pos, S, U = workloads.large_inputs(nagents=nagents, seed=seed)
-    s = LODSolver(n, n, n, ns, DX, "fp32")
-    s.set_stream(stream)
-    s.set_coefficients(Dm, lam, DT)
-    s.set_dirichlet(dvalue, dactive)
-    ones = torch.ones((n, n, n, ns), dtype=torch.float64, device="cuda:0")  # no 16.8 GB host array
-    s.upload(ones)
-    del ones
-    vox = s.voxel_indices(pos)
-    vox[vox < 0] = 0
-    s.set_agent_rates(vox, np.full(nagents, workloads.AGENT_VOLUME), S, U, np.ones((nagents, ns)), DT)
-    gpu_s, wall = time_steps(s, torch, steps, 2)
-    s.close()
-    vs = n ** 3 * ns * steps / gpu_s
-    log(f"c5 fp32: {vs:.4e} vs/s, {gpu_s * 1e3 / steps:.3f} ms/step")
-    return {"workload": f"FP32 {n}^3x{ns}, {nagents} agents", "value": vs, "ms_per_step": gpu_s * 1e3 / steps,
-            "step_hbm_frac": round(n ** 3 * ns * 24 / (gpu_s / steps) / 1e9 / HBM_PEAK_GBS, 4)}
+    out = {"workload": f"{n}^3x{ns}, {nagents} agents"}
+    for prec in precisions:
+        s = LODSolver(n, n, n, ns, DX, prec)
+        s.set_stream(stream)
+        s.set_coefficients(Dm, lam, DT)
+        s.set_dirichlet(dvalue, dactive)
+        ones = torch.ones((n, n, n, ns), dtype=torch.float64, device="cuda:0")  # no 16.8 GB host array
+        s.upload(ones)
+        del ones
+        vox = s.voxel_indices(pos)
+        vox[vox < 0] = 0
+        s.set_agent_rates(vox, np.full(nagents, workloads.AGENT_VOLUME), S, U, np.ones((nagents, ns)), DT)
+        gpu_s, wall = time_steps(s, torch, steps, 2)
+        phases = time_phases(s, torch, count=3)
+        s.close()
+        vs = n ** 3 * ns * steps / gpu_s
+        eb = 8 if prec == "fp64" else 4
+        log(f"c5 {prec}: {vs:.4e} vs/s, {gpu_s * 1e3 / steps:.3f} ms/step, phases {phases}")
+        out[prec] = {"value": vs, "ms_per_step": gpu_s * 1e3 / steps, "phases_ms": phases,
+                     "step_hbm_frac": round(n ** 3 * ns * 6 * eb / (gpu_s / steps) / 1e9 / HBM_PEAK_GBS, 4)}
+    first = out[precisions[0]]
+    out.update(value=first["value"], ms_per_step=first["ms_per_step"], step_hbm_frac=first["step_hbm_frac"])
+    return out
 
 
 def bench_adjoint(torch, stream, n=128, nsteps=50, every=10, nagents=10_000, seed=5):
@@ -330,6 +338,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and adjoint workloads")
     ap.add_argument("--c5", action="store_true", help="also time the FP32 640^3 x 8 workload")
+    ap.add_argument("--c5-only", action="store_true",
+                    help="time only the 640^3 x 8 workload, FP32 and FP64, and print its JSON object")
     ap.add_argument("--batched-only", action="store_true",
                     help="time only the batched workload and print its JSON object (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -353,6 +363,9 @@ def main():
         del a, b
     if args.copy_ceiling:
         print(json.dumps({"copy_ceiling": copy_ceiling(torch), "bytes": N ** 3 * NSUB * 8}), flush=True)
+        return
+    if args.c5_only:
+        print(json.dumps(bench_c5(torch, stream, precisions=("fp32", "fp64"))), flush=True)
         return
     if args.batched_only:
         print(json.dumps(bench_batched(torch, stream)), flush=True)

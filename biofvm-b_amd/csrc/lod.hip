@@ -11,11 +11,11 @@
 // writes the other):
 //  * One wavefront owns 64 lines.  The forward results of the tail of every line are held
 //    in a per-wavefront LDS "line store" (a share of the CU's 160 KB); the backward pass
-//    reads them from there.  The head of a line longer than its store is only swept in the
-//    forward pass, keeping the recurrence value at each segment end in the output row the
-//    backward pass overwrites last; the backward pass then recomputes each head segment
-//    from its checkpoint (re-reading the input, bit-identical arithmetic) into the freed
-//    store.  HBM sees one read and one write per voxel, plus one re-read of the head.
+//    reads them from there.  The part of a line that fits neither the store nor registers
+//    is only swept in the forward pass, keeping the recurrence value at each segment end in
+//    the output row the backward pass overwrites last; the backward pass then recomputes each
+//    such segment from its checkpoint (re-reading the input, bit-identical arithmetic) into
+//    the freed store.  HBM sees one read and one write per voxel, plus that re-read.
 //  * y and z sweeps: lane = line, consecutive lanes on consecutive x (y sweep) or xy
 //    (z sweep), so each step of the recurrence is a coalesced 512 B row access; a ring of
 //    P rows per lane is kept in flight.
@@ -56,7 +56,7 @@ __device__ __forceinline__ gmem<T> as_global(T *p)
     return (gmem<T>)p;
 }
 
-// Checkpoint of the forward recurrence at a head-segment end, kept in output elements that
+// Checkpoint of the forward recurrence at a recomputed segment's start, kept in output elements that
 // the backward pass overwrites later.  FP64 storage keeps the value itself; FP32 storage
 // keeps the double's two 32-bit halves in two elements (off and off2), so that a recomputed
 // segment starts from exactly the value of the first pass, and results do not depend on how
@@ -127,7 +127,8 @@ __device__ __forceinline__ double fwd(double c1, double prev, double v, double d
 }
 
 // First row of a line kept in the line store of L rows: the forward pass keeps [lo, n) in
-// LDS and recomputes the head [0, lo) later; lo is a multiple of P (L >= P is required).
+// LDS; rows before lo go to registers or are recomputed; lo is a multiple of P (L >= P is
+// required).
 __host__ __device__ __forceinline__ int park_boundary(int n, int L, int P)
 {
     if (L >= n)
@@ -274,11 +275,13 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
         const long long lines_per_s = AXIS == 1 ? (long long)nx * nz : (long long)nxy;
         const long long tiles_per_s = (lines_per_s + kWave - 1) / kWave;
         const size_t stride = AXIS == 1 ? (size_t)nx : nxy;
-        // rows [lds_lo, n) go to the line store, [lo, lds_lo) to registers (whole rings of
-        // rows), the head [0, lo) is recomputed
+        // rows [lds_lo, n) go to the line store, the head [0, R) to registers (whole rings of
+        // rows), the middle [R, lds_lo) is recomputed.  The middle is recomputed right after the
+        // tail is substituted back, segment by segment towards the head, so each re-read follows
+        // its first read at the shortest distance the order of the two passes allows.
         const int lds_lo = park_boundary(n, L, P);
         const int nregc = min(RC, lds_lo / P);
-        const int lo = lds_lo - nregc * P;
+        const int R = nregc * P;
         const long long lt = tile - prefix[di];
         const int s = __builtin_amdgcn_readfirstlane((int)(lt / tiles_per_s));  // 64-bit division runs on
                                                                               // the vector unit
@@ -297,12 +300,15 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
         const bool face = act && (x == 0 || x == nx - 1 || o == 0 || o == on - 1);
         const double c1 = as_ctab(D.c1)[s];
 
-        // ---- forward elimination; a ring of P rows is in flight.  The head [0, lo) is only
-        // swept here (its results are recomputed segment by segment in the backward pass,
-        // from checkpoints kept at the segment ends); the tail [lo, n) goes to the line store.
-        const int S = L / P * P;  // head segment length (the host guarantees L >= P)
-        const int nhead = lo > 0 ? (lo + S - 1) / S : 0;
-        const int r0 = lo - (nhead - 1) * S;  // end of the first head segment
+        // ---- forward elimination; a ring of P rows is in flight.  The head [0, R) goes to
+        // registers; the middle [R, lds_lo) is only swept here (its results are recomputed
+        // segment by segment in the backward pass, from checkpoints kept at the segment
+        // starts); the tail [lds_lo, n) goes to the line store.  Measured at 400^3 x 4 against
+        // registers holding the rows just before the tail and the head recomputed last: FP64 y
+        // 0.967 -> 0.946 ms, z 1.024 -> 1.019 ms, FP32 y 0.491 -> 0.475 ms.
+        const int S = L / P * P;  // middle segment length (the host guarantees L >= P)
+        const int nmid = lds_lo > R ? (lds_lo - R + S - 1) / S : 0;
+        const int m0 = lds_lo - (nmid - 1) * S;  // end of the first middle segment
         T buf[P];
 #pragma unroll
         for (int j = 0; j < P; ++j)
@@ -323,7 +329,7 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
         }
         double y = -0.0;
         unsigned akey = 0u;
-        T regs[RC * P];  // register rows [lo, lds_lo)
+        T regs[RC * P];  // register rows [0, R)
         // one full chunk of P rows from k0: prefetches row min(k + P, lim) (PF), stores the
         // results (STORE: 1 = line store at row k - base, 2 = registers rp[j])
         auto chunk = [&](int k0, int base, int lim, auto store_tag, auto exact_tag, auto pf_tag, T *rp) {
@@ -359,18 +365,20 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
         const std::integral_constant<int, 2> reg_store;
         const std::false_type inexact, no_pf;
         const std::true_type exact_div, pf;
-        int k0 = 0;
-        for (int next_ck = r0; k0 < lo; k0 += P) {  // head: checkpoints only
+#pragma unroll
+        for (int q = 0; q < RC; ++q)  // register rows
+            if (q < nregc)
+                chunk(q * P, 0, n - 1, reg_store, inexact, pf, regs + q * P);
+        if (R > 0 && R < lds_lo)  // checkpoint of the first middle segment
+            put_ck<T>(dst, (size_t)(R - 1) * stride, (size_t)(R - 2) * stride, y);
+        int k0 = R;
+        for (int next_ck = m0; k0 < lds_lo; k0 += P) {  // middle: checkpoints only
             chunk(k0, 0, n - 1, no_store, inexact, pf, nullptr);
-            if (k0 + P == next_ck && next_ck < lo) {
+            if (k0 + P == next_ck && next_ck < lds_lo) {
                 put_ck<T>(dst, (size_t)(next_ck - 1) * stride, (size_t)(next_ck - 2) * stride, y);
                 next_ck += S;
             }
         }
-#pragma unroll
-        for (int q = 0; q < RC; ++q)  // register rows
-            if (q < nregc)
-                chunk(lo + q * P, 0, n - 1, reg_store, inexact, pf, regs + q * P);
         for (k0 = lds_lo; k0 + P <= n; k0 += P)  // tail: line store
             chunk(k0, lds_lo, n - 1, store, inexact, pf, nullptr);
         if (k0 < n) {  // the last, partial chunk (uniform per-row conditions)
@@ -387,26 +395,28 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
         bool exact = false;
         if constexpr (sizeof(T) == 8) {
             // a numerator below 2^-960 or not finite somewhere in the tile: redo the forward
-            // pass with true divisions (same results everywhere else); the head recomputation below
+            // pass with true divisions (same results everywhere else); the middle recomputation below
             // then divides the same way
             if (__builtin_expect(__any(akey >= kKeyBad), 0)) {
                 exact = true;
                 y = -0.0;
-                for (int k = 0; k < lo; ++k) {
-                    y = fma(c1, y, (double)src[(size_t)k * stride]) / tab2[k].x;
-                    if (k + 1 >= r0 && k + 1 < lo && (k + 1 - r0) % S == 0)
-                        dst[(size_t)k * stride] = (T)y;
-                }
 #pragma unroll
                 for (int q = 0; q < RC; ++q)
                     if (q < nregc) {
 #pragma unroll
                         for (int j = 0; j < P; ++j) {
-                            const int k = lo + q * P + j;
+                            const int k = q * P + j;
                             y = fma(c1, y, (double)src[(size_t)k * stride]) / tab2[k].x;
                             regs[q * P + j] = (T)y;
                         }
                     }
+                if (R > 0 && R < lds_lo)
+                    dst[(size_t)(R - 1) * stride] = (T)y;
+                for (int k = R; k < lds_lo; ++k) {
+                    y = fma(c1, y, (double)src[(size_t)k * stride]) / tab2[k].x;
+                    if (k + 1 >= m0 && k + 1 < lds_lo && (k + 1 - m0) % S == 0)
+                        dst[(size_t)k * stride] = (T)y;
+                }
                 for (int k = lds_lo; k < n; ++k) {
                     y = fma(c1, y, (double)src[(size_t)k * stride]) / tab2[k].x;
                     ls[(size_t)(k - lds_lo) * kWave + lane] = (T)y;
@@ -445,11 +455,11 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
                 }
             }
         };
-        // head segments, last first: recompute the forward results from the checkpoint at
+        // middle segments, last first: recompute the forward results from the checkpoint at
         // the row before the segment (re-reading the input), then substitute back.  The ring
         // and checkpoint of the next segment are loaded while the current one substitutes back.
         double yck = -0.0;
-        auto seg_begin = [&](int b) { return b > r0 ? b - S : 0; };
+        auto seg_begin = [&](int b) { return b > m0 ? b - S : R; };
         auto prefetch_segment = [&](int b) {
             const int a = seg_begin(b);
             yck = a == 0 ? -0.0 : get_ck<T>(dst, (size_t)(a - 1) * stride, (size_t)(a - 2) * stride);
@@ -457,27 +467,10 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
             for (int j = 0; j < P; ++j)
                 buf[j] = src[(size_t)min(a + j, b - 1) * stride];
         };
-        if (lo > 0)
-            prefetch_segment(lo);
+        if (nmid > 0)
+            prefetch_segment(lds_lo);
         back(n - 2, lds_lo);
-#pragma unroll
-        for (int q = RC - 1; q >= 0; --q)  // register rows, last first
-            if (q < nregc) {
-#pragma unroll
-                for (int jb = P - TB; jb >= 0; jb -= TB) {
-                    double c[TB];
-#pragma unroll
-                    for (int jj = 0; jj < TB; ++jj)
-                        c[jj] = tabc[lo + q * P + jb + jj];
-#pragma unroll
-                    for (int jj = TB - 1; jj >= 0; --jj) {
-                        const int k = lo + q * P + jb + jj;
-                        xv = fma(-c[jj], xv, (double)regs[q * P + jb + jj]);
-                        dst[(size_t)k * stride] = (face || (act && k == 0)) ? dv : (T)xv;
-                    }
-                }
-            }
-        for (int b = lo; b > 0;) {
+        for (int b = lds_lo; b > R;) {
             const int a = seg_begin(b);
             y = yck;
             for (int kk = a; kk < b; kk += P) {
@@ -494,11 +487,28 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
                         chunk(kk, a, b - 1, store, inexact, pf, nullptr);
                 }
             }
-            if (a > 0)
+            if (a > R)
                 prefetch_segment(a);
             back(b - 1, a);
             b = a;
         }
+#pragma unroll
+        for (int q = RC - 1; q >= 0; --q)  // register rows, last first
+            if (q < nregc) {
+#pragma unroll
+                for (int jb = P - TB; jb >= 0; jb -= TB) {
+                    double c[TB];
+#pragma unroll
+                    for (int jj = 0; jj < TB; ++jj)
+                        c[jj] = tabc[q * P + jb + jj];
+#pragma unroll
+                    for (int jj = TB - 1; jj >= 0; --jj) {
+                        const int k = q * P + jb + jj;
+                        xv = fma(-c[jj], xv, (double)regs[q * P + jb + jj]);
+                        dst[(size_t)k * stride] = (face || (act && k == 0)) ? dv : (T)xv;
+                    }
+                }
+            }
         if (act && D.dface != nullptr)  // per-voxel face values (uniform branch)
             yz_face_values<T, AXIS>(D, s, dst, stride, n, x, o, face);
     }
@@ -511,11 +521,12 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
 // Variants (PC chunks in flight, REGC register chunks), chosen per launch by x_variant:
 //  * FP64 (2, 0): at most 256 registers per lane, so that two wavefronts share a SIMD (the x
 //    sweep gains more from a second wavefront hiding its latencies than from a deeper ring;
-//    one wavefront per SIMD with register chunks measured no faster).
+//    one wavefront per SIMD with register chunks measured no faster: FP64 register chunks
+//    beside the FP64 working set spill, 86 to 372 registers per lane for 8 to 12 chunks).
 //  * FP32 at up to four wavefronts per CU (1, 17): one wavefront per SIMD, one chunk in
-//    flight, and the forward results of 17 chunks kept in registers between the recomputed
-//    head and the line store (an FP32 chunk takes 16 registers), so that with the line
-//    store's 8 chunks a 400-long line is recomputed nowhere.  Measured at 400^3 x 4: deeper
+//    flight, and the forward results of the first 17 chunks kept in registers (an FP32
+//    chunk takes 16 registers), so that with the line store's 8 chunks a 400-long line is
+//    recomputed nowhere.  Measured at 400^3 x 4: deeper
 //    rings with fewer register chunks were slower at every step (3 and 15: -10%).
 //  * FP32 (3, 0): two wavefronts per SIMD and no register chunks, three chunks in flight (the
 //    kernel needs ~230 registers per lane without register chunks, so two wavefronts per SIMD
@@ -596,16 +607,18 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
         const int Lp = L + 1;
         const int nch = (nx + W - 1) / W;
         const int Lc = L / W;  // chunks the line store holds (L is a multiple of W)
-        // chunks [cl, nch) live in the line store, [cr, cl) in registers and the head [0, cr)
-        // is recomputed.  With register chunks, cr and cl are multiples of PC, so that chunk c
-        // always sits in ring slot c % PC (the host guarantees Lc >= PC for a line longer
-        // than the store, which keeps cl < nch).
+        // chunks [cl, nch) live in the line store, the head [0, cr) in registers and the middle
+        // [cr, cl) is recomputed (right after the line store's chunks are substituted back, so
+        // that each re-read follows its first read as closely as the two passes allow).  With
+        // register chunks, cr and cl are multiples of PC, so that chunk c always sits in ring
+        // slot c % PC (the host guarantees Lc >= PC for a line longer than the store, which
+        // keeps cl < nch).
         int cl = nch - Lc;
         if constexpr (REGC > 0)
             cl = (cl + PC - 1) / PC * PC;
         if (cl < 0)
             cl = 0;
-        const int cr = cl - min(REGC, cl);
+        const int cr = min(REGC, cl);
         const long long lt = t - prefix[di];
         const int s = __builtin_amdgcn_readfirstlane((int)(lt / tiles_per_s));  // 64-bit division runs on
                                                                               // the vector unit
@@ -682,14 +695,14 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
             }
         };
 
-        // ---- forward.  Head chunks [0, cr) are swept without storing their results (the
-        // backward pass recomputes them segment by segment from checkpoints kept at the
-        // segment ends); the chunks after them go to registers and the line store.
-        const int Sc = Lc;  // head segment length in chunks (the host guarantees L >= W)
-        const int nhead = cr > 0 ? (cr + Sc - 1) / Sc : 0;
-        const int r0c = cr - (nhead - 1) * Sc;  // end of the first head segment
+        // ---- forward.  Head chunks [0, cr) go to registers; the middle [cr, cl) is swept
+        // without storing its results (the backward pass recomputes it segment by segment from
+        // checkpoints kept at the segment starts); the tail [cl, nch) goes to the line store.
+        const int Sc = Lc;  // middle segment length in chunks (the host guarantees L >= W)
+        const int nmid = cl > cr ? (cl - cr + Sc - 1) / Sc : 0;
+        const int r0c = cl - (nmid - 1) * Sc;  // end of the first middle segment
         const size_t my_off = (size_t)min(lane, rows_here - 1) * nx;  // this lane's row
-        // Checkpoint of the head segment starting at chunk ca, in the output tile where that
+        // Checkpoint of the middle segment starting at chunk ca, in the output tile where that
         // segment's back-substitution writes last: in a whole tile the 64 lanes fill the
         // first RPI rows of chunk ca (contiguous row segments: a single scattered element
         // per row would make L2 fetch a line for every 8-byte write); otherwise the lane's
@@ -703,7 +716,7 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
             return full_rows ? ck_off(ca) + (size_t)RPI * nx : my_off + (size_t)ca * W - 2;
         };
         T ring[PC][NI];
-        T regs[REGC > 0 ? REGC * W : 1];  // forward results of chunks [cr, cl)
+        T regs[REGC > 0 ? REGC * W : 1];  // forward results of chunks [0, cr)
         double y = -0.0;
         // chunk c: stage the ring slot, refill it with chunk `next`, then W recurrence steps;
         // results (STORE) 1: to the line store at chunk c - base, 2: to registers rp[j]
@@ -791,7 +804,7 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
                                 fwd_chunk(c, ring[p], min(c + PC, nch - 1), 0, no_store, exact_tag, akey, nullptr);
                             else
                                 fwd_chunk(c, ring[p], min(c + PC, nch - 1), cl, store, exact_tag, akey, nullptr);
-                            if (c + 1 == next_ck && next_ck < cr) {  // checkpoint at a segment end
+                            if (c + 1 == next_ck && next_ck < cl) {  // checkpoint at a segment end
                                 put_ck<T>(dst, ck_off(next_ck), ck_off2(next_ck), y);
                                 next_ck += Sc;
                             }
@@ -799,22 +812,23 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
                     }
                 }
             } else {
-                for (int c0 = 0; c0 < cr; c0 += PC) {  // head (cr is a multiple of PC)
+#pragma unroll
+                for (int q = 0; q < REGC; ++q)  // register chunks: chunk q in ring slot q % PC
+                    if (q < cr)
+                        fwd_chunk(q, ring[q % PC], min(q + PC, nch - 1), 0, reg_store, exact_tag, akey, regs + q * W);
+                if (cr > 0 && cr < cl)  // checkpoint of the first middle segment
+                    put_ck<T>(dst, ck_off(cr), ck_off2(cr), y);
+                for (int c0 = cr; c0 < cl; c0 += PC) {  // middle (cr, cl multiples of PC)
 #pragma unroll
                     for (int p = 0; p < PC; ++p) {
                         const int c = c0 + p;
                         fwd_chunk(c, ring[p], min(c + PC, nch - 1), 0, no_store, exact_tag, akey, nullptr);
-                        if (c + 1 == next_ck && next_ck < cr) {  // checkpoint at a segment end
+                        if (c + 1 == next_ck && next_ck < cl) {  // checkpoint at a segment end
                             put_ck<T>(dst, ck_off(next_ck), ck_off2(next_ck), y);
                             next_ck += Sc;
                         }
                     }
                 }
-#pragma unroll
-                for (int q = 0; q < REGC; ++q)  // register chunks: chunk cr + q in ring slot q % PC
-                    if (cr + q < cl)
-                        fwd_chunk(cr + q, ring[q % PC], min(cr + q + PC, nch - 1), 0, reg_store, exact_tag, akey,
-                                  regs + q * W);
                 for (int c0 = cl; c0 < nch; c0 += PC) {  // line store (cl is a multiple of PC)
 #pragma unroll
                     for (int p = 0; p < PC; ++p) {
@@ -883,11 +897,11 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
             store_chunk(c);
             __syncthreads();
         };
-        // head segments, last first: recompute from the checkpoint (re-reading the input),
+        // middle segments, last first: recompute from the checkpoint (re-reading the input),
         // then substitute back; the next segment's ring and checkpoint are loaded while the
-        // current one substitutes back
+        // current one substitutes back.  The register chunks (the head) come last.
         double yck = -0.0;
-        auto seg_begin = [&](int cb) { return cb > r0c ? cb - Sc : 0; };
+        auto seg_begin = [&](int cb) { return cb > r0c ? cb - Sc : cr; };
         auto prefetch_segment = [&](int cb) {
             const int ca = seg_begin(cb);
             yck = ca == 0 ? -0.0 : get_ck<T>(dst, ck_off(ca), ck_off2(ca));
@@ -897,17 +911,11 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
         };
         const std::false_type from_ls;
         const std::true_type from_regs;
-        if (cr > 0)
-            prefetch_segment(cr);
+        if (cl > cr)
+            prefetch_segment(cl);
         for (int c = nch - 1; c >= cl; --c)
             back_chunk(c, cl, from_ls, nullptr);
-        if constexpr (REGC > 0) {
-#pragma unroll
-            for (int q = REGC - 1; q >= 0; --q)
-                if (cr + q < cl)
-                    back_chunk(cr + q, 0, from_regs, regs + q * W);
-        }
-        for (int cb = cr; cb > 0;) {
+        for (int cb = cl; cb > cr;) {
             const int ca = seg_begin(cb);
             y = yck;
             unsigned unused = 0u;
@@ -924,11 +932,17 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
                     }
                 }
             }
-            if (ca > 0)
+            if (ca > cr)
                 prefetch_segment(ca);
             for (int c = cb - 1; c >= ca; --c)
                 back_chunk(c, ca, from_ls, nullptr);
             cb = ca;
+        }
+        if constexpr (REGC > 0) {
+#pragma unroll
+            for (int q = REGC - 1; q >= 0; --q)
+                if (q < cr)
+                    back_chunk(q, 0, from_regs, regs + q * W);
         }
     }
 }
