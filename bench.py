@@ -121,10 +121,11 @@ def bench_batched(torch, stream, steps=20, seed=3):
     return out
 
 
-def bench_c5(torch, stream, steps=10, nagents=10_000_000, seed=9, precisions=("fp32",)):
+def bench_c5(torch, stream, steps=10, nagents=10_000_000, seed=9, precisions=("fp32",), tune=True):
     """The large workload, 640^3 x 8 substrates (log-spaced D and decay, Dirichlet on
     substrates 0-1, 10M agents in a centred 5 mm sphere), initial field 1: FP32 storage (the
-    bulk path) and, when asked, FP64, with per-phase kernel times."""
+    bulk path) and, when asked, FP64, with per-phase kernel times.  With `tune` the
+    autotuner picks the launch shapes first (its trial launches restore the field)."""
     from biofvmb import LODSolver, workloads
     n, ns = workloads.LARGE["n"], workloads.LARGE["nsub"]
     Dm, lam, dvalue, dactive, pos, S, U = workloads.large_inputs(nagents=nagents, seed=seed)
@@ -140,6 +141,7 @@ def bench_c5(torch, stream, steps=10, nagents=10_000_000, seed=9, precisions=("f
         vox = s.voxel_indices(pos)
         vox[vox < 0] = 0
         s.set_agent_rates(vox, np.full(nagents, workloads.AGENT_VOLUME), S, U, np.ones((nagents, ns)), DT)
+        chosen = s.autotune(reps=2) if tune else None
         gpu_s, wall = time_steps(s, torch, steps, 2)
         phases = time_phases(s, torch, count=3)
         s.close()
@@ -147,6 +149,7 @@ def bench_c5(torch, stream, steps=10, nagents=10_000_000, seed=9, precisions=("f
         eb = 8 if prec == "fp64" else 4
         log(f"c5 {prec}: {vs:.4e} vs/s, {gpu_s * 1e3 / steps:.3f} ms/step, phases {phases}")
         out[prec] = {"value": vs, "ms_per_step": gpu_s * 1e3 / steps, "phases_ms": phases,
+                     "autotune": None if chosen is None else np.asarray(chosen).tolist(),
                      "step_hbm_frac": round(n ** 3 * ns * 6 * eb / (gpu_s / steps) / 1e9 / HBM_PEAK_GBS, 4)}
     first = out[precisions[0]]
     out.update(value=first["value"], ms_per_step=first["ms_per_step"], step_hbm_frac=first["step_hbm_frac"])
@@ -337,7 +340,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and adjoint workloads")
-    ap.add_argument("--c5", action="store_true", help="also time the FP32 640^3 x 8 workload")
+    ap.add_argument("--no-c5", action="store_true", help="skip the FP32 640^3 x 8 workload")
     ap.add_argument("--c5-only", action="store_true",
                     help="time only the 640^3 x 8 workload, FP32 and FP64, and print its JSON object")
     ap.add_argument("--batched-only", action="store_true",
@@ -400,7 +403,7 @@ def main():
         extra["adjoint"] = bench_adjoint(torch, stream)
         extra["agent_moves"] = bench_agent_moves(torch, stream)
         extra["per_call_host_layer"] = bench_per_call()
-    if args.c5:
+    if not args.no_c5 and not args.no_extra:
         extra["fp32_640cube_x8"] = bench_c5(torch, stream)
     cpu = None if args.no_cpu else cpu_baseline()
     line = {

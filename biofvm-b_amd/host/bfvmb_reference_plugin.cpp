@@ -64,6 +64,9 @@ struct PluginState {
     std::vector<uint8_t> agent_active;
     std::vector<double> agent_t1, agent_t2;
     bool agents_uploaded = false;
+    bool agents_resident = false;  // the caller reports agent changes (bfvmb_plugin_invalidate_agents)
+    bool agents_dirty = true;
+    const std::vector<Basic_Agent *> *agents_list = nullptr;
     std::vector<double> bulk[3];  // bulk supply / target / uptake at the last upload
 };
 
@@ -230,52 +233,55 @@ void bfvmb_hip_cell_sources_and_sinks(Microenvironment &M, std::vector<Basic_Age
     PluginState *S = state_for(M);
     if (!S)
         return;
-    // The agents' voxels, flags and constants, gathered in parallel.  An active agent whose
-    // volume changed gets fresh constants first, as simulate_secretion_and_uptake does
-    // (BioFVM_basic_agent.cpp:285-289).  The list goes to the device only when it differs from
-    // the last upload.
+    // The agents' voxels, flags and constants, gathered in parallel straight into the copy of
+    // the last upload (no per-call allocation), noting whether anything changed.  An active
+    // agent whose volume changed gets fresh constants first, as simulate_secretion_and_uptake
+    // does (BioFVM_basic_agent.cpp:285-289).  The list goes to the device only when it differs
+    // from the last upload.
     const long n = (long)agents.size();
     const int ns = S->nsub;
-    std::vector<int> voxel(n);
-    std::vector<double> t1((size_t)n * ns), t2((size_t)n * ns);
-    std::vector<uint8_t> act(n);
-    const bool comparable = S->agents_uploaded && S->agent_voxel.size() == (size_t)n;
-    int changed = comparable ? 0 : 1;
-#pragma omp parallel for schedule(static) reduction(| : changed)
-    for (long i = 0; i < n; ++i) {
+    const bool skip = S->agents_resident && !S->agents_dirty && S->agents_uploaded && S->agents_list == &agents &&
+                      S->agent_voxel.size() == (size_t)n;
+    int changed = S->agents_uploaded && S->agent_voxel.size() == (size_t)n ? 0 : 1;
+    if (S->agent_voxel.size() != (size_t)n) {
+        S->agent_voxel.resize(n);
+        S->agent_active.resize(n);
+        S->agent_t1.resize((size_t)n * ns);
+        S->agent_t2.resize((size_t)n * ns);
+    }
+    int *vox = S->agent_voxel.data();
+    uint8_t *act = S->agent_active.data();
+    double *t1 = S->agent_t1.data(), *t2 = S->agent_t2.data();
+#pragma omp parallel for schedule(static) reduction(| : changed) if (!skip)
+    for (long i = 0; i < (skip ? 0 : n); ++i) {
         Basic_Agent *a = agents[i];
         const bool live = a->*AgentAccess::active();
         if (live && a->*member(VolumeChanged()))
             a->set_internal_uptake_constants(dt);
-        const std::vector<double> &a1 = a->*AgentAccess::t1();
-        const std::vector<double> &a2 = a->*AgentAccess::t2();
-        int v = a->get_current_voxel_index();
-        act[i] = live && v >= 0 ? 1 : 0;
-        voxel[i] = v < 0 ? 0 : v;
-        for (int d = 0; d < ns; ++d) {
-            t1[(size_t)i * ns + d] = a1[d];
-            t2[(size_t)i * ns + d] = a2[d];
-        }
-        if (comparable && !changed) {
-            int c = voxel[i] != S->agent_voxel[i] || act[i] != S->agent_active[i];
-            for (int d = 0; d < ns && !c; ++d)
-                c = std::memcmp(&t1[(size_t)i * ns + d], &S->agent_t1[(size_t)i * ns + d], sizeof(double)) != 0 ||
-                    std::memcmp(&t2[(size_t)i * ns + d], &S->agent_t2[(size_t)i * ns + d], sizeof(double)) != 0;
-            changed |= c;
-        }
+        const double *a1 = (a->*AgentAccess::t1()).data();
+        const double *a2 = (a->*AgentAccess::t2()).data();
+        const int v = a->get_current_voxel_index();
+        const int vv = v < 0 ? 0 : v;
+        const uint8_t on = live && v >= 0 ? 1 : 0;
+        int c = vox[i] != vv || act[i] != on;
+        vox[i] = vv;
+        act[i] = on;
+        double *d1 = t1 + (size_t)i * ns, *d2 = t2 + (size_t)i * ns;
+        c |= std::memcmp(d1, a1, ns * sizeof(double)) != 0 || std::memcmp(d2, a2, ns * sizeof(double)) != 0;
+        std::memcpy(d1, a1, ns * sizeof(double));
+        std::memcpy(d2, a2, ns * sizeof(double));
+        changed |= c;
     }
     if (changed) {
         S->agents_uploaded = false;
-        if (bfvmb_set_agents(S->ctx, (int)n, voxel.data(), t1.data(), t2.data(), act.data()) != BFVMB_OK) {
+        if (bfvmb_set_agents(S->ctx, (int)n, vox, t1, t2, act) != BFVMB_OK) {
             fail("set_agents", S->ctx);
             return;
         }
-        S->agent_voxel.swap(voxel);
-        S->agent_active.swap(act);
-        S->agent_t1.swap(t1);
-        S->agent_t2.swap(t2);
         S->agents_uploaded = true;
     }
+    S->agents_list = &agents;
+    S->agents_dirty = false;
     if (!to_device(M, *S))
         return;
     if (bfvmb_simulate_cell_sources_and_sinks(S->ctx) != BFVMB_OK) {
@@ -338,6 +344,29 @@ void bfvmb_plugin_sync(Microenvironment &M)
     auto it = g_state.find(&M);
     if (it != g_state.end() && it->second.ctx && !it->second.host_current)
         to_host(M, it->second);
+}
+
+void bfvmb_plugin_set_agents_resident(Microenvironment &M, bool resident)
+{
+    PluginState *S = state_for(M);
+    if (!S)
+        return;
+    S->agents_resident = resident;
+    S->agents_dirty = true;
+}
+
+void bfvmb_plugin_invalidate_agents(Microenvironment &M)
+{
+    auto it = g_state.find(&M);
+    if (it != g_state.end())
+        it->second.agents_dirty = true;
+}
+
+void bfvmb_plugin_finish(Microenvironment &M)
+{
+    auto it = g_state.find(&M);
+    if (it != g_state.end() && it->second.ctx && bfvmb_sync(it->second.ctx) != BFVMB_OK)
+        fail("bfvmb_sync", it->second.ctx);
 }
 
 void bfvmb_plugin_invalidate(Microenvironment &M)

@@ -33,6 +33,13 @@ void bfvmb_hip_bulk_sources_and_sinks(BioFVM::Microenvironment &M, double dt);
 void bfvmb_plugin_set_resident(BioFVM::Microenvironment &M, bool resident);
 void bfvmb_plugin_sync(BioFVM::Microenvironment &M);        // device -> p_density_vectors
 void bfvmb_plugin_invalidate(BioFVM::Microenvironment &M);  // host field edited: upload next call
+void bfvmb_plugin_finish(BioFVM::Microenvironment &M);      // wait until the GPU work queued so far is done
+// Agents kept on the device too: bfvmb_hip_cell_sources_and_sinks stops reading the agent list
+// at every call (for a million agents that walk costs ~30 ms, ten times the GPU step) and
+// reuses the last upload until bfvmb_plugin_invalidate_agents(M) reports that agents moved,
+// were added or removed, or changed volume, rates or activity.
+void bfvmb_plugin_set_agents_resident(BioFVM::Microenvironment &M, bool resident);
+void bfvmb_plugin_invalidate_agents(BioFVM::Microenvironment &M);
 void bfvmb_plugin_release(BioFVM::Microenvironment &M);     // free the device context
 // Microenvironment::write_to_matlab, from the device when it holds the newer field
 void bfvmb_plugin_write_to_matlab(BioFVM::Microenvironment &M, const std::string &filename);

@@ -281,7 +281,7 @@ def test_host_layer_moves_match_reference(mode):
         assert np.array_equal(out["fields"][step], f), f"step {step}"
 
 
-@pytest.mark.parametrize("mode", ["hip", "hipres"])
+@pytest.mark.parametrize("mode", ["hip", "hipres", "hipall"])
 def test_reference_driver_moves_with_gpu_slots(mode):
     """The reference's own agents moved by its own routes, with the GPU in its slots: the plugin
     re-uploads the agent list when voxels or activity changed."""

@@ -138,7 +138,7 @@ def test_host_driver(name, mode):
 
 
 @pytest.mark.skipif(not os.path.exists(REF_DRIVER), reason="reference build with the plugin (make -C oracle ref-hip) absent")
-@pytest.mark.parametrize("mode", ["hip", "hipres"])
+@pytest.mark.parametrize("mode", ["hip", "hipres", "hipall"])
 @pytest.mark.parametrize("name", golden.DNODES_NAMES)
 def test_reference_driver_with_gpu_slot(name, mode):
     """The reference's driver sets the nodes through its own Microenvironment; the plugin reads

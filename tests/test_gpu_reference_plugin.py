@@ -23,7 +23,7 @@ DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver_hip")
 
 
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="reference build with the plugin (make -C oracle ref-hip) absent")
-@pytest.mark.parametrize("mode", ["hip", "hipres"])
+@pytest.mark.parametrize("mode", ["hip", "hipres", "hipall"])
 @pytest.mark.parametrize("name", golden.ALL_NAMES)
 def test_reference_driver_with_gpu_slot(name, mode):
     case, ref = golden.load(name)
@@ -40,7 +40,7 @@ def test_reference_driver_with_gpu_slot(name, mode):
 
 
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="reference build with the plugin (make -C oracle ref-hip) absent")
-@pytest.mark.parametrize("mode", ["hip", "hipres"])
+@pytest.mark.parametrize("mode", ["hip", "hipres", "hipall"])
 @pytest.mark.parametrize("name", golden.MAT_NAMES)
 def test_reference_driver_mat_file(name, mode):
     """bfvmb_plugin_write_to_matlab in the reference's own driver (from the device when the
@@ -57,7 +57,7 @@ def test_reference_driver_mat_file(name, mode):
 
 
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="reference build with the plugin (make -C oracle ref-hip) absent")
-@pytest.mark.parametrize("mode", ["hip", "hipres"])
+@pytest.mark.parametrize("mode", ["hip", "hipres", "hipall"])
 @pytest.mark.parametrize("name", golden.BULK_NAMES)
 def test_reference_driver_bulk(name, mode):
     """bfvmb_hip_bulk_sources_and_sinks in place of simulate_bulk_sources_and_sinks in the
