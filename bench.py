@@ -343,6 +343,8 @@ def main():
     ap.add_argument("--no-c5", action="store_true", help="skip the FP32 640^3 x 8 workload")
     ap.add_argument("--c5-only", action="store_true",
                     help="time only the 640^3 x 8 workload, FP32 and FP64, and print its JSON object")
+    ap.add_argument("--c5-defaults", action="store_true",
+                    help="with --c5-only: the library's default launch shapes instead of the autotuner's")
     ap.add_argument("--batched-only", action="store_true",
                     help="time only the batched workload and print its JSON object (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -368,7 +370,7 @@ def main():
         print(json.dumps({"copy_ceiling": copy_ceiling(torch), "bytes": N ** 3 * NSUB * 8}), flush=True)
         return
     if args.c5_only:
-        print(json.dumps(bench_c5(torch, stream, precisions=("fp32", "fp64"))), flush=True)
+        print(json.dumps(bench_c5(torch, stream, precisions=("fp32", "fp64"), tune=not args.c5_defaults)), flush=True)
         return
     if args.batched_only:
         print(json.dumps(bench_batched(torch, stream)), flush=True)
