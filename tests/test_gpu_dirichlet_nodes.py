@@ -75,6 +75,54 @@ def test_capi_fp32_error(name):
         assert r.max() <= 1e-5
 
 
+def test_fp32_x_variants_agree_with_nodes():
+    """Every FP32 x-sweep variant (chunks in flight, register chunks, tables by scalar loads or
+    in LDS) applies the per-voxel values the same way: bit-identical results."""
+    case, ref = golden.load("dnodes_17x23x31x3")
+    print("[gpu] dirichlet nodes: fp32 x-sweep variants", flush=True)
+    outs = []
+    for variant in (1, 2, 3, 4):
+        s = _solver(case, "fp32", launch=None)
+        s.set_launch(0, 0, 4 if variant <= 2 else 6, variant)
+        s.run(case.nsteps)
+        outs.append(s.download())
+        s.close()
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    last = max(ref["fields"])
+    r = np.abs(outs[0] - ref["fields"][last]) / np.maximum(np.abs(ref["fields"][last]), 1e-300)
+    assert r.max() <= 1e-5
+
+
+def test_adjoint_with_nodes_matches_finite_differences():
+    """The adjoint of a run with per-voxel Dirichlet values: the forward record is the
+    reference's run, and directional derivatives in rho_0 and the agent tables agree with
+    central differences of the oracle's map (the face values are constants of the map)."""
+    import oracle as orc
+    case, ref = golden.load("dnodes_sparse_12x9x14x2")
+    print("[gpu] adjoint with dirichlet nodes", flush=True)
+    voxel, t1, t2 = orc.agent_tables(case)
+    tgt = np.random.default_rng(11).uniform(0.0, 2.0, case.init.shape)
+    s = _solver(case, "fp64")
+    s.adjoint_forward(case.nsteps, 2)
+    rT = s.download()
+    assert np.array_equal(rT, ref["fields"][case.nsteps])
+    g0, g1, g2 = s.adjoint_backward(2.0 * (rT - tgt))
+    s.close()
+    rng = np.random.default_rng(5)
+
+    def fd(f, x, v, h):
+        a, b = f(x + h * v), f(x - h * v)
+        return float(np.sum((a - b) * (a + b - 2.0 * tgt))) / (2 * h)
+
+    v = rng.standard_normal(case.init.shape)
+    d0 = fd(lambda r: orc.run(case, rho=r), case.init, v, 1e-3)
+    assert abs(d0 - float(np.sum(g0 * v))) <= 1e-6 * abs(d0)
+    v = rng.standard_normal(t1.shape) * np.abs(t1).max()
+    d1 = fd(lambda x: orc.run(case, tables=(x, t2)), t1, v, 0.5)
+    assert abs(d1 - float(np.sum(g1 * v))) <= 1e-6 * abs(d1)
+
+
 def test_uniform_values_replace_nodes_and_activation_keeps_them():
     """set_dirichlet with values goes back to one value per substrate (as
     set_substrate_dirichlet_value writes every voxel); with value None only the activations
