@@ -72,25 +72,29 @@ def time_steps(s, torch, steps, warmup):
 
 
 def time_phases(s, torch, count=10):
-    """Kernel times inside the step: `count` steps launched phase by phase (agents, x, y, z
-    sweeps) with a HIP event on the library's stream between consecutive kernels, so each
-    kernel runs after the kernel it follows in a step (a sweep launched right after another
-    launch of itself runs 3-7% faster than in the step).  Returns ms per launch of each
-    phase and "step", the sum of the four."""
-    names = ("sources_sinks", "sweep_x", "sweep_y", "sweep_z")
-    s.launch_phase(0, 1)
-    for p in (1, 2, 3):
+    """Kernel times inside the step: `count` steps launched kernel by kernel as `run()` launches
+    them (FP64: the x sweep with the cell sources and sinks in its launch, then the y and z
+    sweeps; FP32: the agent kernel, then the three sweeps) with a HIP event on the library's
+    stream between consecutive kernels, so each kernel runs after the kernel it follows in a
+    step (a sweep launched right after another launch of itself runs 3-7% faster than in the
+    step).  Returns ms per launch of each kernel and "step", their sum."""
+    if s.precision == "fp64":
+        phases = ((4, "sweep_x_with_agents"), (2, "sweep_y"), (3, "sweep_z"))
+    else:
+        phases = ((0, "sources_sinks"), (1, "sweep_x"), (2, "sweep_y"), (3, "sweep_z"))
+    for p, _ in phases:
         s.launch_phase(p, 1)
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4 * count + 1)]
+    n = len(phases)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(n * count + 1)]
     ev[0].record()
     for k in range(count):
-        for p in range(4):
+        for i, (p, _) in enumerate(phases):
             s.launch_phase(p, 1)
-            ev[4 * k + p + 1].record()
+            ev[n * k + i + 1].record()
     torch.cuda.synchronize()
-    out = {n: sum(ev[4 * k + p].elapsed_time(ev[4 * k + p + 1]) for k in range(count)) / count
-           for p, n in enumerate(names)}
-    out["step"] = ev[0].elapsed_time(ev[4 * count]) / count
+    out = {name: sum(ev[n * k + i].elapsed_time(ev[n * k + i + 1]) for k in range(count)) / count
+           for i, (_, name) in enumerate(phases)}
+    out["step"] = ev[0].elapsed_time(ev[n * count]) / count
     return out
 
 
@@ -400,8 +404,12 @@ def main():
         log(f"{prec}: {vs:.4e} vs/s, {gpu_s * 1e3 / args.steps:.3f} ms/step, phases {phases}")
 
     f64 = results["fp64"]
-    dom = max(("sweep_x", "sweep_y", "sweep_z"), key=lambda k: f64["phases_ms"][k])
+    dom = max(("sweep_x_with_agents", "sweep_y", "sweep_z"), key=lambda k: f64["phases_ms"][k])
+    # field read + write; the x sweep's launch also carries the agents' bytes (index, two
+    # coefficients per substrate, voxel read + write per substrate)
     bytes_per_launch = N ** 3 * NSUB * 2 * 8
+    if dom == "sweep_x_with_agents":
+        bytes_per_launch += N_AGENTS * (4 + 2 * NSUB * 8 + 2 * NSUB * 8)
     achieved = bytes_per_launch / (f64["phases_ms"][dom] / 1e3) / 1e9
     traffic, traffic_meta = traffic_record(dom)  # FETCH_SIZE + WRITE_SIZE counters (profiles/)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

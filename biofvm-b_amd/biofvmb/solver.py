@@ -329,6 +329,7 @@ class Batch:
         shapes = [tuple(int(v) for v in s) for s in shapes]
         n = len(shapes)
         self.shapes = shapes
+        self.precision = precision
         nx, ny, nz, ns = (np.ascontiguousarray([s[i] for s in shapes], dtype=np.int32) for i in range(4))
         dxa = np.ascontiguousarray(np.broadcast_to(dx, (n,)), dtype=np.float64)
         h = ctypes.c_void_p()

@@ -44,6 +44,8 @@ struct DomainDesc {
     const int *seg_start;        // agents: [nseg+1] offsets into the voxel-sorted list
     const long long *seg_off;    // agents: [nseg] voxel offset z*nx*ny + y*nx + x
     const double *t1, *t2;       // agents: [nsub][nact] temp1 / temp2, sorted order
+    const int *xseg;             // agents: [x-sweep tiles per substrate + 1] first segment of
+                                 // each tile of 64 rows (z*ny + y), or nullptr: no agents
     const double *bulk_ts;       // bulk: [nsub][nz][ny][nx] target*supply (nullptr: none)
     const double *bulk_us;       // bulk: [nsub][nz][ny][nx] uptake+supply
     double dt;                   // time step frozen by set_coefficients
@@ -101,6 +103,7 @@ struct Domain {
     long long *seg_off = nullptr;
     double *t1 = nullptr, *t2 = nullptr;
     size_t seg_cap = 0, act_cap = 0;  // segments / agents the device arrays have room for
+    int *xseg = nullptr;           // first segment of each x-sweep tile (rebuilt with the descriptors)
     std::vector<int> agent_order;  // sorted position -> index in the caller's agent list (after
                                    // moves: current once domain_agent_order has run)
     std::vector<long long> agent_off;  // device offset of each sorted agent's voxel, as set
@@ -147,6 +150,8 @@ struct Group {
 
 // Launchers (lod.hip, field.hip): one kernel over all domains of the group.
 int launch_sweep(Group *g, int axis);   // field[cur] -> field[cur^1] for every domain
+int launch_sweep_x_agents(Group *g);    // cell sources/sinks, then the x sweep, in one launch
+int launch_xseg(Group *g, const Domain &d);  // xseg from the domain's segment list
 int launch_agents(Group *g);            // in place on field[cur]
 int launch_bulk(Group *g, double dt);   // in place on field[cur], domains with bulk rates;
                                         // dt <= 0: each domain's own time step

@@ -50,6 +50,27 @@ __global__ __launch_bounds__(256) void k_agents(const DomainDesc<T> *__restrict_
     *p = (T)r;
 }
 
+// First segment of each x-sweep tile (64 consecutive rows r = z*ny + y of a substrate): the
+// smallest k with seg_off[k] >= 64 * t * nx, t = 0 .. ntiles (t = ntiles: nseg).  The list is
+// sorted by voxel, so a tile's occupied voxels are the segments [xseg[t], xseg[t + 1]).
+__global__ __launch_bounds__(256) void k_xseg(const long long *__restrict__ seg_off, int nseg, int nx,
+                                              long long nrows, int ntiles, int *__restrict__ xseg)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > ntiles)
+        return;
+    const long long v = min((long long)t * kWave, nrows) * nx;
+    int lo = 0, hi = nseg;  // first k in [lo, hi] with seg_off[k] >= v
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (seg_off[mid] < v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    xseg[t] = lo;
+}
+
 // Bulk sources and sinks (Microenvironment::simulate_bulk_sources_and_sinks,
 // BioFVM_microenvironment.cpp:1110-1153), one thread per (domain, substrate, voxel):
 //   rho = fma(dt, T*S, rho) / (((U + S)*dt) + 1)
@@ -283,6 +304,17 @@ int domain_densities_at_noalloc(Group *g, int i, int n, const int *voxel, double
         (rc = hip_check(g, hipMemcpyAsync(out, tout, obytes, hipMemcpyDeviceToHost, g->stream), "densities")))
         return rc;
     return hip_check(g, hipStreamSynchronize(g->stream), "densities sync");
+}
+
+int launch_xseg(Group *g, const Domain &d)
+{
+    if (d.nseg == 0 || d.xseg == nullptr)
+        return BFVMB_OK;
+    const long long nrows = (long long)d.ny * d.nz;
+    const int ntiles = (int)((nrows + kWave - 1) / kWave);
+    hipLaunchKernelGGL(k_xseg, dim3((unsigned)((ntiles + 1 + 255) / 256)), dim3(256), 0, g->stream, d.seg_off, d.nseg,
+                       d.nx, nrows, ntiles, d.xseg);
+    return hip_check(g, hipGetLastError(), "agent tile index launch");
 }
 
 int launch_agents(Group *g)

@@ -98,6 +98,7 @@ std::vector<DomainDesc<T>> build_descs(Group *g)
         e.seg_off = d.seg_off;
         e.t1 = d.t1;
         e.t2 = d.t2;
+        e.xseg = d.nseg > 0 ? d.xseg : nullptr;
         e.bulk_ts = d.bulk;
         e.bulk_us = d.bulk ? d.bulk + d.nvox * d.nsub : nullptr;
         e.dt = d.dt;
@@ -164,6 +165,8 @@ int sync_descriptors(Group *g)
     if (!rc)
         rc = hip_check(g, hipMemcpyAsync(g->d_prefix, flat.data(), flat.size() * sizeof(long long), hipMemcpyHostToDevice,
                                          g->stream), "prefixes");
+    for (size_t i = 0; i < nd && !rc; ++i)  // the x sweep's tile index of the agent lists
+        rc = launch_xseg(g, g->doms[i]);
     if (!rc)
         rc = hip_check(g, hipStreamSynchronize(g->stream), "descriptor sync");
     if (rc)
@@ -220,6 +223,8 @@ int group_init(Group *g, int ndom, const int *nx, const int *ny, const int *nz, 
             (rc = hip_check(g, hipMalloc(&d.field[1], bytes), "field buffer")) ||
             (rc = hip_check(g, hipMalloc(&d.d_dval, d.nsub * g->elem), "dirichlet values")) ||
             (rc = hip_check(g, hipMalloc((void **)&d.d_dact, d.nsub), "dirichlet flags")) ||
+            (rc = hip_check(g, hipMalloc((void **)&d.xseg, ((d.ny * (size_t)d.nz + kWave - 1) / kWave + 1) * sizeof(int)),
+                            "agent tile index")) ||
             (rc = hip_check(g, hipMemsetAsync(d.field[0], 0, bytes, g->stream), "zero field")) ||
             (rc = upload_dirichlet(g, d)))
             return rc;
@@ -243,6 +248,7 @@ void group_free(Group *g)
         (void)hipFree(d.coef);
         (void)hipFree(d.d_dval);
         (void)hipFree(d.d_dact);
+        (void)hipFree(d.xseg);
         (void)hipFree(d.d_dface);
     }
     adjoint_free(g);
@@ -615,9 +621,17 @@ static int eager_steps(Group *g, int nsteps, bool with_agents)
         // (BioFVM_microenvironment.cpp:1110-1169, BioFVM_parallel.cpp:693-1395)
         if (with_agents && bulk && (rc = launch_bulk(g, 0.0)))
             return rc;
-        if (with_agents && (rc = launch_agents(g)))
+        // FP64: the cell sources and sinks run inside the x sweep, tile by tile, before it
+        // reads each tile (results identical to a separate pass; one launch and one pass over
+        // the occupied voxels fewer).  FP32 keeps the separate pass: its x sweep runs one
+        // wavefront per SIMD, which leaves no other wavefront to cover the update's latency.
+        if (with_agents && g->elem == 8)
+            rc = launch_sweep_x_agents(g);
+        else if (!with_agents || !(rc = launch_agents(g)))
+            rc = launch_sweep(g, 0);
+        if (rc)
             return rc;
-        for (int axis = 0; axis < 3; ++axis)
+        for (int axis = 1; axis < 3; ++axis)
             if ((rc = launch_sweep(g, axis)))
                 return rc;
     }
@@ -667,7 +681,7 @@ int group_steps(Group *g, int nsteps, bool with_agents, bool use_graph)
 
 int group_launch_phase(Group *g, int phase, int count)
 {
-    if (phase < 0 || phase > 3 || count < 0)
+    if (phase < 0 || phase > 4 || count < 0)
         return set_error(g, BFVMB_ERR_ARG, "launch_phase: bad argument");
     if (phase > 0)
         for (const Domain &d : g->doms)
@@ -675,7 +689,7 @@ int group_launch_phase(Group *g, int phase, int count)
                 return set_error(g, BFVMB_ERR_STATE, "set_coefficients has not been called for every domain");
     g->epoch++;
     for (int i = 0; i < count; ++i) {
-        int rc = phase == 0 ? launch_agents(g) : launch_sweep(g, phase - 1);
+        int rc = phase == 0 ? launch_agents(g) : phase == 4 ? launch_sweep_x_agents(g) : launch_sweep(g, phase - 1);
         if (rc)
             return rc;
     }

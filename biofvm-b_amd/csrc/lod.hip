@@ -573,9 +573,15 @@ constexpr int x_waves_per_simd()
     return REGC == 0 ? 2 : 1;
 }
 
+#ifndef BFVMB_XAGENT_BATCHES
+#define BFVMB_XAGENT_BATCHES 4
+#endif
+constexpr int kAgentBatches = BFVMB_XAGENT_BATCHES;  // segment batches of 64 in flight (AG)
+
 // VF: the kernel also applies per-voxel Dirichlet values (descriptor dface) where a domain has
 // them; a separate instantiation, so that the per-substrate case keeps its registers.
-template <class T, int PC, int REGC, bool TL, bool VF>
+// AG: the step's cell sources and sinks are applied first, tile by tile (see below).
+template <class T, int PC, int REGC, bool TL, bool VF, bool AG>
 __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(const DomainDesc<T> *__restrict__ descs,
                                                 const long long *__restrict__ prefix, int ndom, int parity,
                                                 int budget)
@@ -840,6 +846,52 @@ __global__ __launch_bounds__(64, (x_waves_per_simd<T, REGC>())) void k_sweep_x(c
             }
             return akey;
         };
+        // Cell sources and sinks of this tile's rows, before the sweep reads them (the step's
+        // agent update, fused): the occupied voxels of rows [r0, r0 + 64) are the segments
+        // [xseg[tile], xseg[tile + 1]) of the voxel-sorted list.  Each is updated in place in
+        // the input, r = (r + temp1[a]) / temp2[a] over its agents in list order, as the
+        // separate agent kernel does (FP64 arithmetic, one rounding to T); the input is dead
+        // after this sweep and no other tile reads these rows.  kAgentBatches batches of 64
+        // segments are in flight at a time.
+        if constexpr (AG) {
+            if (D.xseg != nullptr) {
+                const int tt = r0 / kWave;
+                const int lo = __builtin_amdgcn_readfirstlane(D.xseg[tt]);
+                const int hi = __builtin_amdgcn_readfirstlane(D.xseg[tt + 1]);
+                T *f = D.field[parity] + (size_t)s * nvox;
+                const double *a1 = D.t1 + (size_t)s * D.nact;
+                const double *a2 = D.t2 + (size_t)s * D.nact;
+                constexpr int NB = kAgentBatches;
+                for (int k0 = lo; k0 < hi; k0 += NB * kWave) {
+                    long long off[NB];
+                    int b0[NB], b1[NB];
+                    double r[NB];
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) {
+                        const int k = k0 + b * kWave + lane;
+                        const bool ok = k < hi;
+                        off[b] = ok ? D.seg_off[k] : -1;
+                        b0[b] = ok ? D.seg_start[k] : 0;
+                        b1[b] = ok ? D.seg_start[k + 1] : 0;
+                    }
+#pragma unroll
+                    for (int b = 0; b < NB; ++b)
+                        r[b] = off[b] >= 0 ? (double)f[off[b]] : 0.0;
+#pragma unroll
+                    for (int b = 0; b < NB; ++b)
+                        for (int a = b0[b]; a < b1[b]; ++a) {
+                            r[b] = r[b] + a1[a];
+                            r[b] = r[b] / a2[a];
+                        }
+#pragma unroll
+                    for (int b = 0; b < NB; ++b)
+                        if (off[b] >= 0)
+                            f[off[b]] = (T)r[b];
+                }
+                __threadfence_block();  // the updates are stored before the sweep loads them
+            }
+        }
+
         bool exact = false;
         const unsigned akey = forward(inexact);
         if constexpr (sizeof(T) == 8) {
@@ -981,45 +1033,49 @@ void launch_yz(const Group *g, const DomainDesc<T> *d, const long long *pre, int
         hipLaunchKernelGGL((k_sweep_yz<T, AXIS, 64>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
 }
 
-template <class T, bool VF>
+template <class T, bool VF, bool AG>
 void launch_x_vf(const Group *g, const DomainDesc<T> *d, const long long *pre, int ndom, unsigned grid, size_t lds,
                  XVariant xv)
 {
     if constexpr (sizeof(T) == 8) {
         (void)xv;
-        hipLaunchKernelGGL((k_sweep_x<T, 2, 0, false, VF>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+        hipLaunchKernelGGL((k_sweep_x<T, 2, 0, false, VF, AG>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
     } else if (xv.pc == 1) {
         if (xv.tl)
-            hipLaunchKernelGGL((k_sweep_x<T, 1, 17, true, VF>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+            hipLaunchKernelGGL((k_sweep_x<T, 1, 17, true, VF, AG>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
         else
-            hipLaunchKernelGGL((k_sweep_x<T, 1, 17, false, VF>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+            hipLaunchKernelGGL((k_sweep_x<T, 1, 17, false, VF, AG>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
     } else {
         if (xv.tl)
-            hipLaunchKernelGGL((k_sweep_x<T, 3, 0, true, VF>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+            hipLaunchKernelGGL((k_sweep_x<T, 3, 0, true, VF, AG>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
         else
-            hipLaunchKernelGGL((k_sweep_x<T, 3, 0, false, VF>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+            hipLaunchKernelGGL((k_sweep_x<T, 3, 0, false, VF, AG>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
     }
 }
 
 template <class T>
 void launch_x(const Group *g, const DomainDesc<T> *d, const long long *pre, int ndom, unsigned grid, size_t lds,
-              XVariant xv)
+              XVariant xv, bool agents)
 {
     bool vf = false;  // some domain has per-voxel Dirichlet values
     for (const Domain &dm : g->doms)
         vf = vf || !dm.dface.empty();
-    if (vf)
-        launch_x_vf<T, true>(g, d, pre, ndom, grid, lds, xv);
+    if (vf && agents)
+        launch_x_vf<T, true, true>(g, d, pre, ndom, grid, lds, xv);
+    else if (vf)
+        launch_x_vf<T, true, false>(g, d, pre, ndom, grid, lds, xv);
+    else if (agents)
+        launch_x_vf<T, false, true>(g, d, pre, ndom, grid, lds, xv);
     else
-        launch_x_vf<T, false>(g, d, pre, ndom, grid, lds, xv);
+        launch_x_vf<T, false, false>(g, d, pre, ndom, grid, lds, xv);
 }
 
-template <bool VF>
+template <bool VF, bool AG>
 hipError_t x_attributes(int bytes)
 {
-    const void *k[5] = {(const void *)k_sweep_x<double, 2, 0, false, VF>, (const void *)k_sweep_x<float, 1, 17, false, VF>,
-                        (const void *)k_sweep_x<float, 1, 17, true, VF>, (const void *)k_sweep_x<float, 3, 0, false, VF>,
-                        (const void *)k_sweep_x<float, 3, 0, true, VF>};
+    const void *k[5] = {(const void *)k_sweep_x<double, 2, 0, false, VF, AG>, (const void *)k_sweep_x<float, 1, 17, false, VF, AG>,
+                        (const void *)k_sweep_x<float, 1, 17, true, VF, AG>, (const void *)k_sweep_x<float, 3, 0, false, VF, AG>,
+                        (const void *)k_sweep_x<float, 3, 0, true, VF, AG>};
     hipError_t e = hipSuccess;
     for (int i = 0; i < 5 && e == hipSuccess; ++i)
         e = hipFuncSetAttribute(k[i], hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
@@ -1027,7 +1083,7 @@ hipError_t x_attributes(int bytes)
 }
 
 template <class T>
-int launch_sweep_t(Group *g, int axis)
+int launch_sweep_t(Group *g, int axis, bool agents)
 {
     int rc = sync_descriptors(g);
     if (rc)
@@ -1109,7 +1165,7 @@ int launch_sweep_t(Group *g, int axis)
     grid = std::min<long long>(grid, 1LL << 30);
     const DomainDesc<T> *d = static_cast<const DomainDesc<T> *>(g->d_desc);
     if (axis == 0)
-        launch_x<T>(g, d, pre, ndom, (unsigned)grid, lds, xv);
+        launch_x<T>(g, d, pre, ndom, (unsigned)grid, lds, xv, agents);
     else if (axis == 1)
         launch_yz<T, 1>(g, d, pre, ndom, (unsigned)grid, lds, ring);
     else
@@ -1137,8 +1193,15 @@ hipError_t yz_attributes(int bytes)
 int launch_sweep(Group *g, int axis)
 {
     if (g->precision == BFVMB_FP64)
-        return launch_sweep_t<double>(g, axis);
-    return launch_sweep_t<float>(g, axis);
+        return launch_sweep_t<double>(g, axis, false);
+    return launch_sweep_t<float>(g, axis, false);
+}
+
+int launch_sweep_x_agents(Group *g)
+{
+    if (g->precision == BFVMB_FP64)
+        return launch_sweep_t<double>(g, 0, true);
+    return launch_sweep_t<float>(g, 0, true);
 }
 
 int sweep_lds_attributes(size_t lds_bytes)
@@ -1153,9 +1216,13 @@ int sweep_lds_attributes(size_t lds_bytes)
     if (e == hipSuccess)
         e = yz_attributes<float, 2>(bytes);
     if (e == hipSuccess)
-        e = x_attributes<false>(bytes);
+        e = x_attributes<false, false>(bytes);
     if (e == hipSuccess)
-        e = x_attributes<true>(bytes);
+        e = x_attributes<true, false>(bytes);
+    if (e == hipSuccess)
+        e = x_attributes<false, true>(bytes);
+    if (e == hipSuccess)
+        e = x_attributes<true, true>(bytes);
     return e == hipSuccess ? 0 : (int)e;
 }
 

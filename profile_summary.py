@@ -63,6 +63,7 @@ def traffic(fetch_db, write_db, out):
         big = max(vals) if vals else None
         cal[label] = COPY_BYTES / big if big else 1.0
     res, rows = {}, []
+    # k_sweep_x<T, PC, REGC, TL, VF, AG>: AG = the step's agent update in the same launch
     phase = {"k_sweep_x": "sweep_x", "k_sweep_yz<double, 1": "sweep_y", "k_sweep_yz<double, 2": "sweep_z"}
     for k in f:
         s = short(k)
@@ -73,6 +74,8 @@ def traffic(fetch_db, write_db, out):
         rows.append((s, fb, wb))
         for pre, ph in phase.items():
             if s.startswith(pre) and "double" in s:
+                if ph == "sweep_x" and s.endswith(", true>"):
+                    ph = "sweep_x_with_agents"
                 res[ph] = fb + wb
     # provenance: the sweep source these counters describe (bench.py reports the traffic only
     # while biofvm-b_amd/csrc/lod.hip still has this hash) and the commit checked out
