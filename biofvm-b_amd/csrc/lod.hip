@@ -158,12 +158,25 @@ __device__ __forceinline__ int find_domain(const long long *prefix, int ndom, lo
     return lo;
 }
 
+// y/z sweeps: Thomas tables read with scalar loads (true) or staged in LDS per substrate.
+// Every lane of a wavefront is at the same row, so the table entries are wave-uniform: read
+// with scalar loads they take no LDS and no vector registers, and the line store gets the
+// 24 bytes per row the staged tables took (+19 rows of 64 lanes at 400 rows, FP64).
+// Measured at 400^3 x 4 (ms, isolated launches): FP64 y 0.980 -> 0.932, z 1.023 -> 0.967
+// (step 3.364 -> 3.231); FP32 y 0.483 -> 0.462, but z 0.479 -> 0.487, so the FP32 z sweep
+// keeps its staged tables.
+template <class T, int AXIS>
+__host__ __device__ constexpr bool yz_scalar_tables() { return sizeof(T) == 8 || AXIS == 1; }
+// LDS bytes per row of a y/z line taken by its staged tables ({den, rden}, cc)
+template <class T, int AXIS>
+__host__ __device__ constexpr long long yz_table_bytes() { return yz_scalar_tables<T, AXIS>() ? 0 : 24; }
+
 // Elements per line kept in the y/z line store for a line of n with `budget` LDS bytes
-// (after the {den, rden}, cc tables of 24 bytes per element).
-template <class T>
+// (after the staged tables, if any).
+template <class T, int AXIS>
 __host__ __device__ __forceinline__ int yz_line_store(int n, long long budget)
 {
-    long long L = (budget - 24LL * n) / (kWave * (long long)sizeof(T));
+    long long L = (budget - yz_table_bytes<T, AXIS>() * n) / (kWave * (long long)sizeof(T));
     if (L > n)
         L = n;
     return L < 0 ? 0 : (int)L;
@@ -192,9 +205,9 @@ __host__ __device__ __forceinline__ int x_line_store(int nx, long long budget, b
 }
 
 // ------------------------------------------------------------------------------ y / z
-// Block = one wavefront = 64 lines of one substrate.  LDS: {den, rden}[n], cc[n] of the
-// block's substrate, then the line store [L][64].  P = rows in flight per lane (ring
-// depth); table entries are read in batches of TB per wait.  Before the line store, up to
+// Block = one wavefront = 64 lines of one substrate.  LDS: the line store [L][64] (staged
+// tables {den, rden}[n], cc[n] of the block's substrate before it, where yz_scalar_tables is
+// off).  P = rows in flight per lane (ring depth); table entries are read in batches of TB.  Before the line store, up to
 // reg_rows<T>() forward results per line are kept in registers (the wavefront has a SIMD to
 // itself at the default four per CU, and its register file is larger than its LDS share).
 // Register rows per lane: 160 in FP64 (320 registers, room made by table batches of 8), 320 in
@@ -202,8 +215,11 @@ __host__ __device__ __forceinline__ int x_line_store(int nx, long long budget, b
 // 400-long line stays on chip.  Measured at 400^3 x 4: FP32 y 0.522 -> 0.499 ms, z 0.542 ->
 // 0.505 ms against 128 FP32 rows; FP64 y 0.972 -> 0.961, z 1.029 -> 1.012 against 128 rows and
 // batches of 16.
+#ifndef BFVMB_YZ_REG_ROWS_FP64
+#define BFVMB_YZ_REG_ROWS_FP64 160
+#endif
 template <class T>
-constexpr int reg_rows() { return sizeof(T) == 8 ? 160 : 320; }
+constexpr int reg_rows() { return sizeof(T) == 8 ? BFVMB_YZ_REG_ROWS_FP64 : 320; }
 template <class T>
 constexpr int table_batch() { return sizeof(T) == 8 ? 8 : 16; }
 
@@ -270,8 +286,9 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
         const int n = AXIS == 1 ? ny : nz;
         double2 *tab2 = reinterpret_cast<double2 *>(smem);
         double *tabc = reinterpret_cast<double *>(tab2 + n);
-        T *ls = reinterpret_cast<T *>(tabc + n);
-        const int L = yz_line_store<T>(n, budget);
+        constexpr bool ST = yz_scalar_tables<T, AXIS>();
+        T *ls = ST ? reinterpret_cast<T *>(smem) : reinterpret_cast<T *>(tabc + n);
+        const int L = yz_line_store<T, AXIS>(n, budget);
         const long long lines_per_s = AXIS == 1 ? (long long)nx * nz : (long long)nxy;
         const long long tiles_per_s = (lines_per_s + kWave - 1) / kWave;
         const size_t stride = AXIS == 1 ? (size_t)nx : nxy;
@@ -315,7 +332,12 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
             buf[j] = src[(size_t)min(j, n - 1) * stride];
         // stage this substrate's Thomas tables after the first ring of rows is on its way, so
         // that the two load latencies overlap (a block starts with both at every new tile)
-        if (di != tab_dom || s != tab_s) {
+        const ctab sden = as_ctab(D.tab[AXIS].den + (size_t)s * n);
+        const ctab srden = as_ctab(D.tab[AXIS].rden + (size_t)s * n);
+        const ctab scc = as_ctab(D.tab[AXIS].cc + (size_t)s * n);
+        auto tdr = [&](int k) -> double2 { return ST ? make_double2(sden[k], srden[k]) : tab2[k]; };
+        auto tcc = [&](int k) -> double { return ST ? scc[k] : tabc[k]; };
+        if (!ST && (di != tab_dom || s != tab_s)) {
             __syncthreads();
             const double *gd = D.tab[AXIS].den + (size_t)s * n, *gr = D.tab[AXIS].rden + (size_t)s * n;
             const double *gc = D.tab[AXIS].cc + (size_t)s * n;
@@ -341,7 +363,7 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
                 double2 t[TB];
 #pragma unroll
                 for (int jj = 0; jj < TB; ++jj)
-                    t[jj] = tab2[k0 + jb + jj];
+                    t[jj] = tdr(k0 + jb + jj);
 #pragma unroll
                 for (int jj = 0; jj < TB; ++jj) {
                     const int j = jb + jj;
@@ -386,7 +408,7 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
             for (int j = 0; j < P; ++j) {
                 const int k = k0 + j;
                 if (k < n) {
-                    const double2 t = tab2[k];
+                    const double2 t = tdr(k);
                     y = fwd<T>(c1, y, (double)buf[j], t.x, t.y, akey);
                     ls[(size_t)(k - lds_lo) * kWave + lane] = (T)y;
                 }
@@ -406,19 +428,19 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
 #pragma unroll
                         for (int j = 0; j < P; ++j) {
                             const int k = q * P + j;
-                            y = fma(c1, y, (double)src[(size_t)k * stride]) / tab2[k].x;
+                            y = fma(c1, y, (double)src[(size_t)k * stride]) / tdr(k).x;
                             regs[q * P + j] = (T)y;
                         }
                     }
                 if (R > 0 && R < lds_lo)
                     dst[(size_t)(R - 1) * stride] = (T)y;
                 for (int k = R; k < lds_lo; ++k) {
-                    y = fma(c1, y, (double)src[(size_t)k * stride]) / tab2[k].x;
+                    y = fma(c1, y, (double)src[(size_t)k * stride]) / tdr(k).x;
                     if (k + 1 >= m0 && k + 1 < lds_lo && (k + 1 - m0) % S == 0)
                         dst[(size_t)k * stride] = (T)y;
                 }
                 for (int k = lds_lo; k < n; ++k) {
-                    y = fma(c1, y, (double)src[(size_t)k * stride]) / tab2[k].x;
+                    y = fma(c1, y, (double)src[(size_t)k * stride]) / tdr(k).x;
                     ls[(size_t)(k - lds_lo) * kWave + lane] = (T)y;
                 }
             }
@@ -435,7 +457,7 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
 #pragma unroll
                     for (int j = 0; j < TB; ++j) {
                         v[j] = (double)ls[(size_t)(kt - j - base) * kWave + lane];
-                        c[j] = tabc[kt - j];
+                        c[j] = tcc(kt - j);
                     }
 #pragma unroll
                     for (int j = 0; j < TB; ++j) {
@@ -448,7 +470,7 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
                     for (int j = 0; j < TB; ++j) {
                         const int k = kt - j;
                         if (k >= base) {
-                            xv = fma(-tabc[k], xv, (double)ls[(size_t)(k - base) * kWave + lane]);
+                            xv = fma(-tcc(k), xv, (double)ls[(size_t)(k - base) * kWave + lane]);
                             dst[(size_t)k * stride] = (face || (act && k == 0)) ? dv : (T)xv;
                         }
                     }
@@ -500,7 +522,7 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
                     double c[TB];
 #pragma unroll
                     for (int jj = 0; jj < TB; ++jj)
-                        c[jj] = tabc[q * P + jb + jj];
+                        c[jj] = tcc(q * P + jb + jj);
 #pragma unroll
                     for (int jj = TB - 1; jj >= 0; --jj) {
                         const int k = q * P + jb + jj;
@@ -1115,8 +1137,9 @@ int launch_sweep_t(Group *g, int axis, bool agents)
             floor = std::min(full, stg + (size_t)kWave * (chunks * W + 1) * sizeof(T));
         } else {
             const int n = axis == 1 ? d.ny : d.nz;
-            full = (size_t)n * 24 + (size_t)kWave * n * sizeof(T);
-            floor = std::min(full, (size_t)n * 24 + (size_t)kWave * 16 * sizeof(T));
+            const size_t tb = (size_t)(axis == 1 ? yz_table_bytes<T, 1>() : yz_table_bytes<T, 2>());
+            full = (size_t)n * tb + (size_t)kWave * n * sizeof(T);
+            floor = std::min(full, (size_t)n * tb + (size_t)kWave * 16 * sizeof(T));
         }
         if (floor > g->max_lds_per_block)
             return set_error(g, BFVMB_ERR_ARG, "sweep: axis too long for the LDS line store");
@@ -1139,7 +1162,7 @@ int launch_sweep_t(Group *g, int axis, bool agents)
         int minL = 1 << 30;
         for (const Domain &d : g->doms) {
             const int n = axis == 1 ? d.ny : d.nz;
-            const int L = yz_line_store<T>(n, (long long)lds);
+            const int L = axis == 1 ? yz_line_store<T, 1>(n, (long long)lds) : yz_line_store<T, 2>(n, (long long)lds);
             if (L < n)
                 minL = std::min(minL, L);
         }
