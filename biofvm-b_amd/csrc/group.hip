@@ -167,10 +167,25 @@ int sync_descriptors(Group *g)
                                          g->stream), "prefixes");
     for (size_t i = 0; i < nd && !rc; ++i)  // the x sweep's tile index of the agent lists
         rc = launch_xseg(g, g->doms[i]);
+    // the densest tile decides whether the step fuses the agent update into the x sweep
+    std::vector<std::vector<int>> xs(nd);
+    for (size_t i = 0; i < nd && !rc; ++i) {
+        const Domain &d = g->doms[i];
+        if (d.nseg == 0)
+            continue;
+        xs[i].resize((d.ny * (size_t)d.nz + kWave - 1) / kWave + 1);
+        rc = hip_check(g, hipMemcpyAsync(xs[i].data(), d.xseg, xs[i].size() * sizeof(int), hipMemcpyDeviceToHost,
+                                         g->stream), "agent tile index");
+    }
     if (!rc)
         rc = hip_check(g, hipStreamSynchronize(g->stream), "descriptor sync");
     if (rc)
         return rc;
+    int densest = 0;
+    for (const auto &v : xs)
+        for (size_t t = 0; t + 1 < v.size(); ++t)
+            densest = std::max(densest, v[t + 1] - v[t]);
+    g->fuse_agents = g->elem == 8 && densest <= kFuseMaxSegsPerTile;
     g->desc_dirty = false;
     clear_graphs(g);  // captured launches hold the old pointers
     return BFVMB_OK;
@@ -612,6 +627,15 @@ int group_bulk(Group *g, double dt)
     return launch_bulk(g, dt);
 }
 
+// The step's cell sources and sinks and its x sweep: one launch (fuse_agents) or two.
+static int launch_agents_and_x(Group *g)
+{
+    if (g->fuse_agents)
+        return launch_sweep_x_agents(g);
+    int rc = launch_agents(g);
+    return rc ? rc : launch_sweep(g, 0);
+}
+
 static int eager_steps(Group *g, int nsteps, bool with_agents)
 {
     const bool bulk = g->prefix[4].back() > 0;
@@ -622,14 +646,10 @@ static int eager_steps(Group *g, int nsteps, bool with_agents)
         if (with_agents && bulk && (rc = launch_bulk(g, 0.0)))
             return rc;
         // FP64: the cell sources and sinks run inside the x sweep, tile by tile, before it
-        // reads each tile (results identical to a separate pass; one launch and one pass over
-        // the occupied voxels fewer).  FP32 keeps the separate pass: its x sweep runs one
-        // wavefront per SIMD, which leaves no other wavefront to cover the update's latency.
-        if (with_agents && g->elem == 8)
-            rc = launch_sweep_x_agents(g);
-        else if (!with_agents || !(rc = launch_agents(g)))
-            rc = launch_sweep(g, 0);
-        if (rc)
+        // reads each tile (results identical to a separate pass; one launch fewer), unless a
+        // tile is dense (kFuseMaxSegsPerTile).  FP32 keeps the separate pass: its x sweep runs
+        // one wavefront per SIMD, which leaves no other wavefront to cover the update's latency.
+        if ((rc = with_agents ? launch_agents_and_x(g) : launch_sweep(g, 0)))
             return rc;
         for (int axis = 1; axis < 3; ++axis)
             if ((rc = launch_sweep(g, axis)))
@@ -689,7 +709,7 @@ int group_launch_phase(Group *g, int phase, int count)
                 return set_error(g, BFVMB_ERR_STATE, "set_coefficients has not been called for every domain");
     g->epoch++;
     for (int i = 0; i < count; ++i) {
-        int rc = phase == 0 ? launch_agents(g) : phase == 4 ? launch_sweep_x_agents(g) : launch_sweep(g, phase - 1);
+        int rc = phase == 0 ? launch_agents(g) : phase == 4 ? launch_agents_and_x(g) : launch_sweep(g, phase - 1);
         if (rc)
             return rc;
     }

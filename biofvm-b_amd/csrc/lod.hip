@@ -164,19 +164,23 @@ __device__ __forceinline__ int find_domain(const long long *prefix, int ndom, lo
 // 24 bytes per row the staged tables took (+19 rows of 64 lanes at 400 rows, FP64).
 // Measured at 400^3 x 4 (ms, isolated launches): FP64 y 0.980 -> 0.932, z 1.023 -> 0.967
 // (step 3.364 -> 3.231); FP32 y 0.483 -> 0.462, but z 0.479 -> 0.487, so the FP32 z sweep
-// keeps its staged tables.
+// keeps its staged tables.  Longer FP64 lines do better with staged tables (640^3 x 8: y 8.37
+// against 8.74 ms, z 8.75 against 9.07; 15 KB of tables per substrate, which the scalar cache
+// holds less well), so an FP64 launch reads its tables with scalar loads only where every
+// line of the launch is at most kYzScalarMaxRows long; the FP32 y sweep gains at 640 too
+// (3.32 against 3.64 ms).
+constexpr int kYzScalarMaxRows = 512;
 template <class T, int AXIS>
-__host__ __device__ constexpr bool yz_scalar_tables() { return sizeof(T) == 8 || AXIS == 1; }
+__host__ __device__ constexpr bool yz_scalar_ok() { return sizeof(T) == 8 || AXIS == 1; }
 // LDS bytes per row of a y/z line taken by its staged tables ({den, rden}, cc)
-template <class T, int AXIS>
-__host__ __device__ constexpr long long yz_table_bytes() { return yz_scalar_tables<T, AXIS>() ? 0 : 24; }
+__host__ __device__ constexpr long long yz_table_bytes(bool st) { return st ? 0 : 24; }
 
 // Elements per line kept in the y/z line store for a line of n with `budget` LDS bytes
 // (after the staged tables, if any).
-template <class T, int AXIS>
-__host__ __device__ __forceinline__ int yz_line_store(int n, long long budget)
+template <class T>
+__host__ __device__ __forceinline__ int yz_line_store(int n, long long budget, bool st)
 {
-    long long L = (budget - yz_table_bytes<T, AXIS>() * n) / (kWave * (long long)sizeof(T));
+    long long L = (budget - yz_table_bytes(st) * n) / (kWave * (long long)sizeof(T));
     if (L > n)
         L = n;
     return L < 0 ? 0 : (int)L;
@@ -206,8 +210,8 @@ __host__ __device__ __forceinline__ int x_line_store(int nx, long long budget, b
 
 // ------------------------------------------------------------------------------ y / z
 // Block = one wavefront = 64 lines of one substrate.  LDS: the line store [L][64] (staged
-// tables {den, rden}[n], cc[n] of the block's substrate before it, where yz_scalar_tables is
-// off).  P = rows in flight per lane (ring depth); table entries are read in batches of TB.  Before the line store, up to
+// tables {den, rden}[n], cc[n] of the block's substrate before it, in launches that do not
+// read them with scalar loads, ST false).  P = rows in flight per lane (ring depth); table entries are read in batches of TB.  Before the line store, up to
 // reg_rows<T>() forward results per line are kept in registers (the wavefront has a SIMD to
 // itself at the default four per CU, and its register file is larger than its LDS share).
 // Register rows per lane: 160 in FP64 (320 registers, room made by table batches of 8), 320 in
@@ -254,7 +258,7 @@ __device__ __attribute__((noinline)) void yz_face_values(const DomainDesc<T> &D,
     }
 }
 
-template <class T, int AXIS, int P>
+template <class T, int AXIS, int P, bool ST>
 __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict__ descs,
                                                  const long long *__restrict__ prefix, int ndom, int parity,
                                                  int budget)
@@ -286,9 +290,8 @@ __global__ __launch_bounds__(64) void k_sweep_yz(const DomainDesc<T> *__restrict
         const int n = AXIS == 1 ? ny : nz;
         double2 *tab2 = reinterpret_cast<double2 *>(smem);
         double *tabc = reinterpret_cast<double *>(tab2 + n);
-        constexpr bool ST = yz_scalar_tables<T, AXIS>();
         T *ls = ST ? reinterpret_cast<T *>(smem) : reinterpret_cast<T *>(tabc + n);
-        const int L = yz_line_store<T, AXIS>(n, budget);
+        const int L = yz_line_store<T>(n, budget, ST);
         const long long lines_per_s = AXIS == 1 ? (long long)nx * nz : (long long)nxy;
         const long long tiles_per_s = (lines_per_s + kWave - 1) / kWave;
         const size_t stride = AXIS == 1 ? (size_t)nx : nxy;
@@ -1043,16 +1046,43 @@ Shape shape_of(const Group *g, int axis)
     return sh;
 }
 
+template <class T, int AXIS, bool ST>
+void launch_yz_st(const Group *g, const DomainDesc<T> *d, const long long *pre, int ndom, unsigned grid, size_t lds,
+                  int ring)
+{
+    if (ring <= 16)
+        hipLaunchKernelGGL((k_sweep_yz<T, AXIS, 16, ST>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+    else if (ring <= 32)
+        hipLaunchKernelGGL((k_sweep_yz<T, AXIS, 32, ST>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+    else
+        hipLaunchKernelGGL((k_sweep_yz<T, AXIS, 64, ST>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+}
+
+// whether a y/z launch over the group reads its tables with scalar loads
+template <class T, int AXIS>
+bool yz_scalar(const Group *g)
+{
+    if (!yz_scalar_ok<T, AXIS>())
+        return false;
+    if (sizeof(T) == 4)
+        return true;
+    for (const Domain &dm : g->doms)
+        if ((AXIS == 1 ? dm.ny : dm.nz) > kYzScalarMaxRows)
+            return false;
+    return true;
+}
+
 template <class T, int AXIS>
 void launch_yz(const Group *g, const DomainDesc<T> *d, const long long *pre, int ndom, unsigned grid, size_t lds,
                int ring)
 {
-    if (ring <= 16)
-        hipLaunchKernelGGL((k_sweep_yz<T, AXIS, 16>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
-    else if (ring <= 32)
-        hipLaunchKernelGGL((k_sweep_yz<T, AXIS, 32>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
-    else
-        hipLaunchKernelGGL((k_sweep_yz<T, AXIS, 64>), dim3(grid), dim3(kWave), lds, g->stream, d, pre, ndom, g->cur, (int)lds);
+    if constexpr (yz_scalar_ok<T, AXIS>()) {
+        if (yz_scalar<T, AXIS>(g)) {
+            launch_yz_st<T, AXIS, true>(g, d, pre, ndom, grid, lds, ring);
+            return;
+        }
+    }
+    launch_yz_st<T, AXIS, false>(g, d, pre, ndom, grid, lds, ring);
 }
 
 template <class T, bool VF, bool AG>
@@ -1082,12 +1112,17 @@ void launch_x(const Group *g, const DomainDesc<T> *d, const long long *pre, int 
     bool vf = false;  // some domain has per-voxel Dirichlet values
     for (const Domain &dm : g->doms)
         vf = vf || !dm.dface.empty();
-    if (vf && agents)
-        launch_x_vf<T, true, true>(g, d, pre, ndom, grid, lds, xv);
-    else if (vf)
+    if constexpr (sizeof(T) == 8) {  // the agent update runs in FP64 steps only
+        if (agents) {
+            if (vf)
+                launch_x_vf<T, true, true>(g, d, pre, ndom, grid, lds, xv);
+            else
+                launch_x_vf<T, false, true>(g, d, pre, ndom, grid, lds, xv);
+            return;
+        }
+    }
+    if (vf)
         launch_x_vf<T, true, false>(g, d, pre, ndom, grid, lds, xv);
-    else if (agents)
-        launch_x_vf<T, false, true>(g, d, pre, ndom, grid, lds, xv);
     else
         launch_x_vf<T, false, false>(g, d, pre, ndom, grid, lds, xv);
 }
@@ -1137,7 +1172,7 @@ int launch_sweep_t(Group *g, int axis, bool agents)
             floor = std::min(full, stg + (size_t)kWave * (chunks * W + 1) * sizeof(T));
         } else {
             const int n = axis == 1 ? d.ny : d.nz;
-            const size_t tb = (size_t)(axis == 1 ? yz_table_bytes<T, 1>() : yz_table_bytes<T, 2>());
+            const size_t tb = (size_t)yz_table_bytes(axis == 1 ? yz_scalar<T, 1>(g) : yz_scalar<T, 2>(g));
             full = (size_t)n * tb + (size_t)kWave * n * sizeof(T);
             floor = std::min(full, (size_t)n * tb + (size_t)kWave * 16 * sizeof(T));
         }
@@ -1162,7 +1197,7 @@ int launch_sweep_t(Group *g, int axis, bool agents)
         int minL = 1 << 30;
         for (const Domain &d : g->doms) {
             const int n = axis == 1 ? d.ny : d.nz;
-            const int L = axis == 1 ? yz_line_store<T, 1>(n, (long long)lds) : yz_line_store<T, 2>(n, (long long)lds);
+            const int L = yz_line_store<T>(n, (long long)lds, axis == 1 ? yz_scalar<T, 1>(g) : yz_scalar<T, 2>(g));
             if (L < n)
                 minL = std::min(minL, L);
         }
@@ -1200,14 +1235,24 @@ int launch_sweep_t(Group *g, int axis, bool agents)
     return BFVMB_OK;
 }
 
+template <class T, int AXIS, bool ST>
+hipError_t yz_attributes_st(int bytes)
+{
+    hipError_t e = hipFuncSetAttribute((const void *)k_sweep_yz<T, AXIS, 16, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)k_sweep_yz<T, AXIS, 32, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)k_sweep_yz<T, AXIS, 64, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    return e;
+}
+
 template <class T, int AXIS>
 hipError_t yz_attributes(int bytes)
 {
-    hipError_t e = hipFuncSetAttribute((const void *)k_sweep_yz<T, AXIS, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_sweep_yz<T, AXIS, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_sweep_yz<T, AXIS, 64>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    hipError_t e = yz_attributes_st<T, AXIS, false>(bytes);
+    if constexpr (yz_scalar_ok<T, AXIS>())
+        if (e == hipSuccess)
+            e = yz_attributes_st<T, AXIS, true>(bytes);
     return e;
 }
 
@@ -1222,9 +1267,9 @@ int launch_sweep(Group *g, int axis)
 
 int launch_sweep_x_agents(Group *g)
 {
-    if (g->precision == BFVMB_FP64)
-        return launch_sweep_t<double>(g, 0, true);
-    return launch_sweep_t<float>(g, 0, true);
+    if (g->precision != BFVMB_FP64)
+        return set_error(g, BFVMB_ERR_STATE, "the fused agent update is an FP64 kernel");
+    return launch_sweep_t<double>(g, 0, true);
 }
 
 int sweep_lds_attributes(size_t lds_bytes)
@@ -1242,10 +1287,12 @@ int sweep_lds_attributes(size_t lds_bytes)
         e = x_attributes<false, false>(bytes);
     if (e == hipSuccess)
         e = x_attributes<true, false>(bytes);
+    if (e == hipSuccess)  // FP64 x sweep with the agent update
+        e = hipFuncSetAttribute((const void *)k_sweep_x<double, 2, 0, false, false, true>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e == hipSuccess)
-        e = x_attributes<false, true>(bytes);
-    if (e == hipSuccess)
-        e = x_attributes<true, true>(bytes);
+        e = hipFuncSetAttribute((const void *)k_sweep_x<double, 2, 0, false, true, true>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     return e == hipSuccess ? 0 : (int)e;
 }
 

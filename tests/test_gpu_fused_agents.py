@@ -4,7 +4,9 @@ launch phase 4): bit-identical to the separate agent kernel followed by the x sw
 FP32, and in FP64 to the oracle.  Cases: dense agents (several batches of 256 segments per
 tile), several agents per voxel (list order), agents on the x faces of active Dirichlet
 substrates (their update is overwritten, as in the reference), row counts that leave a
-partial tile, domains without agents beside domains with them, agents that move."""
+partial tile, domains without agents beside domains with them, agents that move, and tiles
+denser than the fusion limit (2048 occupied voxels per tile: the step then launches the agent
+kernel and the x sweep separately, with the same results)."""
 import numpy as np
 import pytest
 
@@ -28,12 +30,14 @@ def _case(name):
                                np.array([c.ny, c.nz]) * c.dx - 1e-9)
     elif name == "wide_150x70x3x2":  # 210 rows, long lines, sparse agents
         c = cf.make_case(150, 70, 3, 2, seed=43, nagents=400, nsteps=2, dactive=[0, 1])
+    elif name == "dense_tiles_128x70x2x2":  # ~6000 occupied voxels per tile of 64 rows
+        c = cf.make_case(128, 70, 2, 2, seed=45, nagents=16000, nsteps=2, shared_voxels=True, dactive=[1, 0])
     else:
         raise KeyError(name)
     return c
 
 
-NAMES = ["dense_33x21x13x3", "xfaces_40x9x8x2", "wide_150x70x3x2"]
+NAMES = ["dense_33x21x13x3", "xfaces_40x9x8x2", "wide_150x70x3x2", "dense_tiles_128x70x2x2"]
 
 
 def _solver(c, precision):
