@@ -12,7 +12,7 @@
 //   set_total_volume / rates / set_internal_uptake_constants, then per step
 //   simulate_cell_sources_and_sinks(dt) followed by simulate_diffusion_decay(...).
 //
-// usage: ref_driver <case.bin> <out.bin> [lod|avx] [granularity] [--time] [--mat out.mat]
+// usage: ref_driver <case.bin> <out.bin> [lod|avx] [granularity] [--time|--time-pipelined] [--mat out.mat]
 //                   [--bulk rates.bin] [--grad grad.bin] [--dnodes nodes.bin] [--serial-agents]
 //                   [--moves moves.bin moves_out.bin]
 //        ref_driver_hip <case.bin> <out.bin> [hip|hipres|hipall] ...   (BFVMB_PLUGIN build: the
@@ -82,12 +82,16 @@ void bulk_uptake(Microenvironment *, int v, double *dst)
 int main(int argc, char **argv)
 {
     if (argc < 3) {
-        fprintf(stderr, "usage: %s case.bin out.bin [lod|avx] [granularity] [--time]\n", argv[0]);
+        fprintf(stderr, "usage: %s case.bin out.bin [lod|avx] [granularity] [--time|--time-pipelined]\n", argv[0]);
         return 2;
     }
     std::string solver = argc > 3 ? argv[3] : "lod";
     int gran = argc > 4 ? atoi(argv[4]) : 1;
-    bool timing = argc > 5 && std::string(argv[5]) == "--time";
+    // --time: wall time per step, each GPU step finished inside it; --time-pipelined: the GPU
+    // work is finished after the first and the last step only, so the host queues later
+    // steps while earlier ones run (ms_per_later_step is then the throughput of steps 2..n)
+    const bool pipelined = argc > 5 && std::string(argv[5]) == "--time-pipelined";
+    bool timing = pipelined || (argc > 5 && std::string(argv[5]) == "--time");
     std::string mat_path;  // --mat PATH: Microenvironment::write_to_matlab after the last step
     std::string grad_path;  // --grad PATH: compute_all_gradient_vectors after the last step,
                             // written as [voxel][substrate][3] doubles
@@ -342,7 +346,7 @@ int main(int argc, char **argv)
 #endif
         M.simulate_diffusion_decay(dt, size, rank, coords, dims, cart);
 #ifdef BFVMB_PLUGIN
-        if (hip && timing)
+        if (hip && timing && (!pipelined || s == 1 || s == nsteps))
             bfvmb_plugin_finish(M);  // the step's GPU work inside the timed region
 #endif
         auto t_b = std::chrono::steady_clock::now();
