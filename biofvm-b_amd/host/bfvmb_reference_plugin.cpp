@@ -68,6 +68,11 @@ struct PluginState {
     bool agents_dirty = true;
     const std::vector<Basic_Agent *> *agents_list = nullptr;
     std::vector<double> bulk[3];  // bulk supply / target / uptake at the last upload
+    // With the field resident, a cell update is queued until the next solver call, which then
+    // runs it and the LOD step together (bfvmb_run: in FP64 one launch for the update and the
+    // x sweep); every other entry point runs a queued update first.  Not with bulk rates set,
+    // since bfvmb_run would apply them again.
+    bool agents_pending = false;
 };
 
 std::map<Microenvironment *, PluginState> g_state;
@@ -176,6 +181,17 @@ bool to_device(Microenvironment &M, PluginState &S)
     return true;
 }
 
+// runs a queued cell update (bfvmb_simulate_cell_sources_and_sinks, as if it had not waited)
+bool flush_agents(PluginState &S)
+{
+    if (!S.agents_pending)
+        return true;
+    S.agents_pending = false;
+    if (bfvmb_simulate_cell_sources_and_sinks(S.ctx) != BFVMB_OK)
+        return fail("simulate_cell_sources_and_sinks", S.ctx);
+    return true;
+}
+
 bool to_host(Microenvironment &M, PluginState &S)
 {
     if (bfvmb_download_density(S.ctx, M.p_density_vectors->data()) != BFVMB_OK)
@@ -219,7 +235,13 @@ void bfvmb_hip_LOD_3D(Microenvironment &M, double dt, int mpi_Size, int mpi_Rank
     S->rescan_dirichlet = false;
     if (!to_device(M, *S))
         return;
-    if (bfvmb_simulate_diffusion_decay(S->ctx, 1) != BFVMB_OK) {
+    if (S->agents_pending) {  // the queued cell update and this step, as bfvmb_run launches them
+        S->agents_pending = false;
+        if (bfvmb_run(S->ctx, 1) != BFVMB_OK) {
+            fail("run", S->ctx);
+            return;
+        }
+    } else if (bfvmb_simulate_diffusion_decay(S->ctx, 1) != BFVMB_OK) {
         fail("simulate_diffusion_decay", S->ctx);
         return;
     }
@@ -231,7 +253,7 @@ void bfvmb_hip_LOD_3D(Microenvironment &M, double dt, int mpi_Size, int mpi_Rank
 void bfvmb_hip_cell_sources_and_sinks(Microenvironment &M, std::vector<Basic_Agent *> &agents, double dt)
 {
     PluginState *S = state_for(M);
-    if (!S)
+    if (!S || !flush_agents(*S))
         return;
     // The agents' voxels, flags and constants, gathered in parallel straight into the copy of
     // the last upload (no per-call allocation), noting whether anything changed.  An active
@@ -284,6 +306,11 @@ void bfvmb_hip_cell_sources_and_sinks(Microenvironment &M, std::vector<Basic_Age
     S->agents_dirty = false;
     if (!to_device(M, *S))
         return;
+    if (S->resident && S->bulk[0].empty()) {  // queued for the next solver call
+        S->agents_pending = true;
+        S->host_current = false;
+        return;
+    }
     if (bfvmb_simulate_cell_sources_and_sinks(S->ctx) != BFVMB_OK) {
         fail("simulate_cell_sources_and_sinks", S->ctx);
         return;
@@ -296,7 +323,7 @@ void bfvmb_hip_cell_sources_and_sinks(Microenvironment &M, std::vector<Basic_Age
 void bfvmb_hip_bulk_sources_and_sinks(Microenvironment &M, double dt)
 {
     PluginState *S = state_for(M);
-    if (!S)
+    if (!S || !flush_agents(*S))
         return;
     // the rates the bulk_*_function slots give now, voxel by voxel, as the reference
     // evaluates them on every call (BioFVM_microenvironment.cpp:1122-1128); uploaded when
@@ -332,7 +359,7 @@ void bfvmb_hip_bulk_sources_and_sinks(Microenvironment &M, double dt)
 void bfvmb_plugin_set_resident(Microenvironment &M, bool resident)
 {
     PluginState *S = state_for(M);
-    if (!S)
+    if (!S || !flush_agents(*S))
         return;
     if (!resident && !S->host_current && S->device_current)
         to_host(M, *S);
@@ -342,14 +369,14 @@ void bfvmb_plugin_set_resident(Microenvironment &M, bool resident)
 void bfvmb_plugin_sync(Microenvironment &M)
 {
     auto it = g_state.find(&M);
-    if (it != g_state.end() && it->second.ctx && !it->second.host_current)
+    if (it != g_state.end() && it->second.ctx && flush_agents(it->second) && !it->second.host_current)
         to_host(M, it->second);
 }
 
 void bfvmb_plugin_set_agents_resident(Microenvironment &M, bool resident)
 {
     PluginState *S = state_for(M);
-    if (!S)
+    if (!S || !flush_agents(*S))
         return;
     S->agents_resident = resident;
     S->agents_dirty = true;
@@ -358,21 +385,21 @@ void bfvmb_plugin_set_agents_resident(Microenvironment &M, bool resident)
 void bfvmb_plugin_invalidate_agents(Microenvironment &M)
 {
     auto it = g_state.find(&M);
-    if (it != g_state.end())
+    if (it != g_state.end() && flush_agents(it->second))
         it->second.agents_dirty = true;
 }
 
 void bfvmb_plugin_finish(Microenvironment &M)
 {
     auto it = g_state.find(&M);
-    if (it != g_state.end() && it->second.ctx && bfvmb_sync(it->second.ctx) != BFVMB_OK)
+    if (it != g_state.end() && it->second.ctx && flush_agents(it->second) && bfvmb_sync(it->second.ctx) != BFVMB_OK)
         fail("bfvmb_sync", it->second.ctx);
 }
 
 void bfvmb_plugin_invalidate(Microenvironment &M)
 {
     auto it = g_state.find(&M);
-    if (it != g_state.end()) {
+    if (it != g_state.end() && flush_agents(it->second)) {
         it->second.device_current = false;
         it->second.rescan_dirichlet = true;
     }
@@ -383,6 +410,8 @@ void bfvmb_plugin_write_to_matlab(Microenvironment &M, const std::string &filena
     // a field newer on the device is written from there (same bytes as
     // Microenvironment::write_to_matlab, BioFVM_microenvironment.cpp:1079-1104)
     auto it = g_state.find(&M);
+    if (it != g_state.end() && it->second.ctx && !flush_agents(it->second))
+        return;
     if (it != g_state.end() && it->second.ctx && !it->second.host_current && it->second.device_current) {
         const double lo[3] = {M.mesh.bounding_box[0], M.mesh.bounding_box[1], M.mesh.bounding_box[2]};
         if (bfvmb_write_matlab(it->second.ctx, filename.c_str(), lo) != BFVMB_OK)
