@@ -16,7 +16,9 @@
 // the six faces, per-voxel values of add_dirichlet_node / update_dirichlet_node included) at
 // every call while the field is synced each call, and in resident mode at the first call and
 // after bfvmb_plugin_invalidate(M).  The agent list goes to the device only when a voxel,
-// active flag or source/sink constant differs from the last upload.
+// active flag or source/sink constant differs from the last upload.  In resident mode the
+// cell update waits for the next bfvmb_hip_LOD_3D call, which runs both together (FP64: the
+// update inside the x sweep's launch); every other call here runs a waiting update first.
 #pragma once
 
 #include <string>
@@ -35,7 +37,7 @@ void bfvmb_plugin_sync(BioFVM::Microenvironment &M);        // device -> p_densi
 void bfvmb_plugin_invalidate(BioFVM::Microenvironment &M);  // host field edited: upload next call
 void bfvmb_plugin_finish(BioFVM::Microenvironment &M);      // wait until the GPU work queued so far is done
 // Agents kept on the device too: bfvmb_hip_cell_sources_and_sinks stops reading the agent list
-// at every call (for a million agents that walk costs ~30 ms, ten times the GPU step) and
+// at every call (for a million agents that walk costs ~2.6 ms on 16 host threads) and
 // reuses the last upload until bfvmb_plugin_invalidate_agents(M) reports that agents moved,
 // were added or removed, or changed volume, rates or activity.
 void bfvmb_plugin_set_agents_resident(BioFVM::Microenvironment &M, bool resident);
