@@ -164,12 +164,16 @@ __device__ __forceinline__ int find_domain(const long long *prefix, int ndom, lo
 // 24 bytes per row the staged tables took (+19 rows of 64 lanes at 400 rows, FP64).
 // Measured at 400^3 x 4 (ms, isolated launches): FP64 y 0.980 -> 0.932, z 1.023 -> 0.967
 // (step 3.364 -> 3.231); FP32 y 0.483 -> 0.462, but z 0.479 -> 0.487, so the FP32 z sweep
-// keeps its staged tables.  Longer FP64 lines do better with staged tables (640^3 x 8: y 8.37
-// against 8.74 ms, z 8.75 against 9.07; 15 KB of tables per substrate, which the scalar cache
-// holds less well), so an FP64 launch reads its tables with scalar loads only where every
-// line of the launch is at most kYzScalarMaxRows long; the FP32 y sweep gains at 640 too
-// (3.32 against 3.64 ms).
-constexpr int kYzScalarMaxRows = 512;
+// keeps its staged tables.  The gain is the line store's: it needs lines that overflow the
+// on-chip storage.  Isolated y / z launches of n^3 domains (us, scalar against staged): FP64
+// n = 128: 28.4 / 29.1 against 26.4 / 26.5; n = 320: 295.5 / 302.9 against 277.6 / 288.9;
+// n = 400: 512 / 541 against 533 / 552; n = 512: 563 / 597 against 573 / 614; at 640^3 x 8
+// staged tables win (y 8.37 against 8.74 ms, z 8.75 against 9.07: 15 KB of tables per
+// substrate, which the scalar cache holds less well).  FP32 y: n = 128: 21.4 against 18.3;
+// n = 400: 259.8 against 265.6; 640^3 x 8: 3.32 against 3.64 ms.  So a launch reads its
+// tables with scalar loads where its longest line has at least kYzScalarMinRows rows (FP64:
+// and at most kYzScalarMaxRows).
+constexpr int kYzScalarMinRows = 352, kYzScalarMaxRows = 512;
 template <class T, int AXIS>
 __host__ __device__ constexpr bool yz_scalar_ok() { return sizeof(T) == 8 || AXIS == 1; }
 // LDS bytes per row of a y/z line taken by its staged tables ({den, rden}, cc)
@@ -1064,12 +1068,10 @@ bool yz_scalar(const Group *g)
 {
     if (!yz_scalar_ok<T, AXIS>())
         return false;
-    if (sizeof(T) == 4)
-        return true;
+    int longest = 0;
     for (const Domain &dm : g->doms)
-        if ((AXIS == 1 ? dm.ny : dm.nz) > kYzScalarMaxRows)
-            return false;
-    return true;
+        longest = std::max(longest, AXIS == 1 ? dm.ny : dm.nz);
+    return longest >= kYzScalarMinRows && (sizeof(T) == 4 || longest <= kYzScalarMaxRows);
 }
 
 template <class T, int AXIS>
