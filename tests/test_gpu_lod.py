@@ -239,3 +239,17 @@ def test_fp64_nonfinite_values_as_the_reference(shape):
     assert np.array_equal(np.isnan(got), np.isnan(ref))
     fin = ~np.isnan(ref)
     assert np.array_equal(got[fin], ref[fin])
+
+
+@pytest.mark.parametrize("shape", [(6, 351, 3, 2), (5, 360, 4, 1), (4, 3, 400, 2), (4, 520, 3, 1), (3, 4, 600, 1)])
+def test_yz_table_modes_by_line_length(shape):
+    """The y/z sweeps read their Thomas tables with scalar loads or from LDS depending on the
+    launch's longest line (352-512 rows in FP64, 352 or more in the FP32 y sweep): lines on
+    either side of both limits, with agents, FP64 bit for bit against the oracle and FP32
+    within 1e-5 of it."""
+    nx, ny, nz, nsub = shape
+    case = cf.make_case(nx, ny, nz, nsub, seed=51, nagents=60, nsteps=3, shared_voxels=True)
+    print(f"[gpu] y/z table modes {shape}", flush=True)
+    ref = orc.run(case)
+    assert np.array_equal(gpu_run(case, "fp64"), ref)
+    assert rel_err(gpu_run(case, "fp32"), ref)[0] <= 1e-5
