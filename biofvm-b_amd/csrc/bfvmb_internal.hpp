@@ -158,10 +158,11 @@ int launch_xseg(Group *g, const Domain &d);  // xseg from the domain's segment l
 // Occupied voxels per x-sweep tile above which the agent update runs as its own kernel: the
 // fused update is one wavefront per tile working through its segments 256 at a time, so a
 // dense tile serialises where the separate kernel spreads the work over the whole GPU.
-// Measured: 400^3 x 4 with 1M agents (at most 1369 per tile) fused 3.36 against 3.41 ms per
-// FP64 step; 640^3 x 8 with 10M agents (up to 4639) fused x sweep 12.30 ms against 9.72 + 2.26
-// separate.
-constexpr int kFuseMaxSegsPerTile = 2048;
+// Measured, FP64 step at 400^3 x 4 with agents in a 3 mm sphere (densest tile; fused against
+// separate, ms): 1M agents (1369) 3.36 against 3.41; 2M (2579) 3.329 against 3.348; 3M
+// (3788) 3.398 against 3.389; 5M (5785) 3.573 against 3.430.  640^3 x 8 with 10M agents
+// (4639): fused x sweep 12.30 ms against 9.72 + 2.26 separate.
+constexpr int kFuseMaxSegsPerTile = 3072;
 int launch_agents(Group *g);            // in place on field[cur]
 int launch_bulk(Group *g, double dt);   // in place on field[cur], domains with bulk rates;
                                         // dt <= 0: each domain's own time step

@@ -167,7 +167,7 @@ int bfvmb_simulate_diffusion_decay(bfvmb_ctx *ctx, int nsteps);
 /* nsteps of the demo time loop (complete_diffusion_demo.cpp:489-493): per step, cell
  * sources/sinks then one LOD step (preceded by the bulk update when bulk rates are set).
  * In FP64 the sources/sinks run inside the x sweep's launch unless some x-sweep tile (64
- * rows y,z) holds more than 2048 occupied voxels; either way the results are those of the
+ * rows y,z) holds more than 3072 occupied voxels; either way the results are those of the
  * two separate calls, bit for bit. */
 int bfvmb_run(bfvmb_ctx *ctx, int nsteps);
 
@@ -181,7 +181,7 @@ int bfvmb_sync(bfvmb_ctx *ctx);
 
 /* One kernel of the step, for measurement: phase 0 = sources/sinks, 1 = x, 2 = y, 3 = z
  * sweep, 4 = sources/sinks and x sweep as bfvmb_run launches them (FP64 with no x-sweep tile
- * of more than 2048 occupied voxels: one fused launch; otherwise phases 0 and 1); enqueued
+ * of more than 3072 occupied voxels: one fused launch; otherwise phases 0 and 1); enqueued
  * `count` times on the context's stream. */
 int bfvmb_launch_phase(bfvmb_ctx *ctx, int phase, int count);
 

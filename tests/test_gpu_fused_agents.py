@@ -5,7 +5,7 @@ FP32, and in FP64 to the oracle.  Cases: dense agents (several batches of 256 se
 tile), several agents per voxel (list order), agents on the x faces of active Dirichlet
 substrates (their update is overwritten, as in the reference), row counts that leave a
 partial tile, domains without agents beside domains with them, agents that move, and tiles
-denser than the fusion limit (2048 occupied voxels per tile: the step then launches the agent
+denser than the fusion limit (3072 occupied voxels per tile: the step then launches the agent
 kernel and the x sweep separately, with the same results)."""
 import numpy as np
 import pytest
