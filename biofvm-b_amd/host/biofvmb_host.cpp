@@ -156,6 +156,7 @@ void Microenvironment::resize_densities(int n)
         bfvmb_destroy(ctx_);
         ctx_ = nullptr;
     }
+    agents_pending_ = false;
     host_stale_ = false;
     host_dirty_ = true;
     dirichlet_dirty_ = true;
@@ -192,6 +193,7 @@ void Microenvironment::resize_space_uniform(double x_start, double x_end, double
         bfvmb_destroy(ctx_);
         ctx_ = nullptr;
     }
+    agents_pending_ = false;
     host_stale_ = false;
     host_dirty_ = true;
     dirichlet_dirty_ = true;
@@ -270,9 +272,18 @@ void Microenvironment::push_dirichlet()
     dirichlet_dirty_ = false;
 }
 
+void Microenvironment::flush_agents()
+{
+    if (!agents_pending_)
+        return;
+    agents_pending_ = false;
+    check(bfvmb_simulate_cell_sources_and_sinks(ctx_), "bfvmb_simulate_cell_sources_and_sinks");
+}
+
 void Microenvironment::sync_to_device()
 {
     ensure_context();
+    flush_agents();
     if (host_dirty_) {
         check(bfvmb_upload_density(ctx_, host_.data()), "bfvmb_upload_density");
         host_dirty_ = false;
@@ -282,6 +293,7 @@ void Microenvironment::sync_to_device()
 
 void Microenvironment::sync_to_host()
 {
+    flush_agents();
     if (ctx_ && host_stale_) {
         check(bfvmb_download_density(ctx_, host_.data()), "bfvmb_download_density");
         host_stale_ = false;
@@ -315,7 +327,7 @@ void Microenvironment::update_rates()
 
 void Microenvironment::simulate_bulk_sources_and_sinks(double dt)
 {
-    sync_to_device();
+    sync_to_device();  // runs a waiting cell update first
     update_rates();
     check(bfvmb_simulate_bulk_sources_and_sinks(ctx_, dt), "bfvmb_simulate_bulk_sources_and_sinks");
     host_stale_ = true;
@@ -323,14 +335,15 @@ void Microenvironment::simulate_bulk_sources_and_sinks(double dt)
 
 void Microenvironment::write_to_matlab(const std::string &filename)
 {
-    sync_to_device();
+    sync_to_device();  // runs a waiting cell update first
     const double lo[3] = {x0_, y0_, z0_};
     check(bfvmb_write_matlab(ctx_, filename.c_str(), lo), "bfvmb_write_matlab");
 }
 
 void Microenvironment::read_from_matlab(const std::string &filename)
 {
-    sync_to_device();  // host edits first, so that a refused file leaves a consistent field
+    sync_to_device();  // host edits (and a waiting cell update) first, so that a refused file
+                       // leaves a consistent field
     const double lo[3] = {x0_, y0_, z0_};
     check(bfvmb_read_matlab(ctx_, filename.c_str(), lo), "bfvmb_read_matlab");
     host_stale_ = true;
@@ -359,6 +372,7 @@ std::vector<double> Microenvironment::nearest_density_vectors(const std::vector<
         }
         return out;
     }
+    flush_agents();
     check(bfvmb_densities_at(ctx_, (int)voxels.size(), voxels.data(), out.data(), 0), "bfvmb_densities_at");
     return out;
 }
@@ -377,13 +391,19 @@ void Microenvironment::ensure_solver(double dt)
 
 void Microenvironment::simulate_diffusion_decay(double dt)
 {
+    const bool agents = agents_pending_;  // ensure_solver would run it on its own
+    agents_pending_ = false;
     ensure_solver(dt);
-    check(bfvmb_simulate_diffusion_decay(ctx_, 1), "bfvmb_simulate_diffusion_decay");
+    if (agents)  // the waiting cell update and this step, as bfvmb_run launches them
+        check(bfvmb_run(ctx_, 1), "bfvmb_run");
+    else
+        check(bfvmb_simulate_diffusion_decay(ctx_, 1), "bfvmb_simulate_diffusion_decay");
     host_stale_ = true;
 }
 
 void Microenvironment::upload_agents(std::vector<Basic_Agent *> &agents, double dt)
 {
+    flush_agents();  // a waiting update uses the list on the device
     // an agent whose volume changed gets new constants first (BioFVM_basic_agent.cpp:285-289);
     // the list is walked only while some agent has such a change pending
     if (g_volume_pending > 0)
@@ -418,13 +438,17 @@ void Microenvironment::simulate_cell_sources_and_sinks(std::vector<Basic_Agent *
     ensure_context();
     sync_to_device();
     upload_agents(agents, dt);
-    check(bfvmb_simulate_cell_sources_and_sinks(ctx_), "bfvmb_simulate_cell_sources_and_sinks");
     host_stale_ = true;
+    if (bulk_[0].empty()) {  // waits for the next simulate_diffusion_decay
+        agents_pending_ = true;
+        return;
+    }
+    check(bfvmb_simulate_cell_sources_and_sinks(ctx_), "bfvmb_simulate_cell_sources_and_sinks");
 }
 
 void Microenvironment::run(std::vector<Basic_Agent *> &agents, double dt, int nsteps)
 {
-    ensure_solver(dt);
+    ensure_solver(dt);  // runs a waiting cell update first
     upload_agents(agents, dt);
     check(bfvmb_run(ctx_, nsteps), "bfvmb_run");
     host_stale_ = true;

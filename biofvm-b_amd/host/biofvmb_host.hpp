@@ -163,7 +163,12 @@ public:
     // nsteps of: sources/sinks then LOD (complete_diffusion_demo.cpp:489-493), on device
     void run(std::vector<Basic_Agent *> &agents, double dt, int nsteps);
 
-    bfvmb_ctx *context() { return ctx_; }
+    // the context, for direct C-ABI calls (a waiting cell update runs first)
+    bfvmb_ctx *context()
+    {
+        flush_agents();
+        return ctx_;
+    }
     int precision() const { return precision_; }
 
 private:
@@ -172,6 +177,7 @@ private:
     void ensure_solver(double dt);
     void upload_agents(std::vector<Basic_Agent *> &agents, double dt);
     void push_dirichlet();
+    void flush_agents();
 
     int precision_, device_;
     int nx_ = 1, ny_ = 1, nz_ = 1;
@@ -189,6 +195,11 @@ private:
     bool agents_uploaded_ = false;
     unsigned long long agents_epoch_ = 0;
     std::vector<double> bulk_[3];  // rates at the last upload
+    // simulate_cell_sources_and_sinks leaves its update waiting for the next
+    // simulate_diffusion_decay, which runs both as bfvmb_run does (FP64: the update inside the
+    // x sweep's launch); every other method that reaches the device runs a waiting update
+    // first.  Not once bulk rates are on the device (bfvmb_run would apply them again).
+    bool agents_pending_ = false;
     bfvmb_ctx *ctx_ = nullptr;
 };
 
