@@ -152,6 +152,9 @@ int main(int argc, char **argv)
     const char *mat_path = nullptr;   // --mat PATH: write_to_matlab after the last step
     const char *bulk_path = nullptr;  // --bulk PATH: supply, target, uptake [nvox][nsub] doubles
     const char *sample_path = nullptr;  // --sample PATH: densities at the agents' voxels per snapshot
+    // --cells-sample PATH (per-call mode): the same densities right after every step's cell
+    // update, before its LOD step (the host layer then runs the waiting update on its own)
+    const char *cells_sample_path = nullptr;
     const char *dnodes_path = nullptr;  // --dnodes PATH: n, voxel[n], values[n][nsub] Dirichlet nodes
     const char *moves_path = nullptr, *moves_out = nullptr;  // --moves IN OUT (the reference driver's format)
     for (int i = 3; i < argc; ++i) {
@@ -170,6 +173,8 @@ int main(int argc, char **argv)
             bulk_path = argv[++i];
         else if (!std::strcmp(argv[i], "--sample") && i + 1 < argc)
             sample_path = argv[++i];
+        else if (!std::strcmp(argv[i], "--cells-sample") && i + 1 < argc)
+            cells_sample_path = argv[++i];
         else if (!std::strcmp(argv[i], "fp32"))
             precision = BFVMB_FP32;
         else if (!std::strcmp(argv[i], "--fused"))
@@ -373,6 +378,16 @@ int main(int argc, char **argv)
                         M.simulate_bulk_sources_and_sinks(dt);
                     if (with_agents)
                         M.simulate_cell_sources_and_sinks(dt);
+                    if (cells_sample_path) {
+                        FILE *cf = std::fopen(cells_sample_path, done + i == 0 ? "wb" : "ab");
+                        if (!cf) {
+                            std::perror(cells_sample_path);
+                            return 3;
+                        }
+                        const std::vector<double> d = M.nearest_density_vectors(voxel);
+                        std::fwrite(d.data(), sizeof(double), d.size(), cf);
+                        std::fclose(cf);
+                    }
                     M.simulate_diffusion_decay(dt);
                 }
             }

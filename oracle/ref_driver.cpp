@@ -112,6 +112,8 @@ int main(int argc, char **argv)
             moves_path = argv[i + 1];
             moves_out = argv[i + 2];
         }
+    std::string cells_path;  // --cells-out PATH: p_density_vectors right after every step's cell
+                             // update, before its LOD step (the GPU slot: after bfvmb_plugin_sync)
     std::string dnodes_path;  // --dnodes PATH: n, voxel[n], values[n][nsub]: per-voxel Dirichlet
                               // values set through the four public node setters in turn
     for (int i = 3; i + 1 < argc; ++i) {
@@ -123,6 +125,8 @@ int main(int argc, char **argv)
             bulk_path = argv[i + 1];
         if (std::string(argv[i]) == "--grad")
             grad_path = argv[i + 1];
+        if (std::string(argv[i]) == "--cells-out")
+            cells_path = argv[i + 1];
     }
 
     int provided = 0;
@@ -337,6 +341,16 @@ int main(int argc, char **argv)
                 M.simulate_cell_sources_and_sinks(dt);
                 omp_set_num_threads(threads);
             }
+        }
+        if (!cells_path.empty() && with_agents) {
+#ifdef BFVMB_PLUGIN
+            if (hip)
+                bfvmb_plugin_sync(M);  // runs a cell update the plugin holds back, then downloads
+#endif
+            FILE *cf = fopen(cells_path.c_str(), s == 1 ? "wb" : "ab");
+            if (!cf) { perror(cells_path.c_str()); return 2; }
+            fwrite(M.p_density_vectors->data(), sizeof(double), M.p_density_vectors->size(), cf);
+            fclose(cf);
         }
 #ifdef BFVMB_PLUGIN
         if ((solver == "hipres" || solver == "hipall") && s == 1)

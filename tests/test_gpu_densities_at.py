@@ -83,3 +83,28 @@ def test_host_layer_samples_match_snapshots(name):
         field = out["fields"][step].reshape(-1, case.nsub)
         assert np.array_equal(samples[i], field[out["voxel"]]), f"{name} step {step}"
         assert np.array_equal(out["fields"][step], ref["fields"][step])
+
+
+@pytest.mark.parametrize("name", ["aniso_17x23x31x3", "multi_24x20x28x5"])
+def test_host_layer_densities_right_after_the_cell_update(name):
+    """Per-call host layer: simulate_cell_sources_and_sinks leaves its update waiting for the
+    next simulate_diffusion_decay; nearest_density_vectors in between runs it first.  The
+    densities at the agents' voxels after every step's cell update are the oracle's, bit for
+    bit, and the run still ends on the reference's field."""
+    case, ref = golden.load(name)
+    print(f"[gpu] host layer densities after the cell update {name}", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        cpath, opath, spath = (os.path.join(d, f) for f in ("case.bin", "out.bin", "cells.bin"))
+        case.write(cpath)
+        subprocess.run([DRIVER, cpath, opath, "fp64", "--cells-sample", spath], check=True, timeout=120)
+        out = cf.read_output(opath, case)
+        samples = np.fromfile(spath, dtype="<f8").reshape(case.nsteps, case.nagents, case.nsub)
+    voxel, t1, t2 = orc.agent_tables(case)
+    active = np.ones(case.nagents, np.uint8)
+    rho = case.init.copy()
+    for k in range(case.nsteps):
+        orc.sources_sinks(rho, np.maximum(voxel, 0), active, t1, t2)
+        assert np.array_equal(samples[k], rho.reshape(-1, case.nsub)[out["voxel"]]), f"{name} step {k + 1}"
+        rho = orc.run(case, nsteps=1, rho=rho, with_agents=False)
+    last = max(ref["fields"])
+    assert np.array_equal(out["fields"][last], ref["fields"][last])

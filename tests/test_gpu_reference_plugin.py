@@ -73,3 +73,29 @@ def test_reference_driver_bulk(name, mode):
         out = cf.read_output(opath, case)
     for step, f in ref["fields"].items():
         assert np.array_equal(out["fields"][step], f), f"{name} step {step}"
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="reference build with the plugin (make -C oracle ref-hip) absent")
+@pytest.mark.parametrize("mode", ["hipres", "hipall"])
+@pytest.mark.parametrize("name", ["aniso_17x23x31x3", "multi_24x20x28x5"])
+def test_reference_driver_field_right_after_the_cell_update(name, mode):
+    """With the field resident the plugin holds each cell update back until the solver call;
+    bfvmb_plugin_sync in between runs it first.  The reference's own driver reads
+    p_density_vectors right after every step's cell update: the GPU slots give the
+    reference's CPU path's fields there, bit for bit, and the same final fields."""
+    case, ref = golden.load(name)
+    print(f"[gpu] reference driver + plugin, field after the cell update {name} {mode}", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        cpath = os.path.join(d, "case.bin")
+        case.write(cpath)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        cells = {}
+        for m in ("lod", mode):
+            opath, kpath = os.path.join(d, f"out_{m}.bin"), os.path.join(d, f"cells_{m}.bin")
+            subprocess.run([DRIVER, cpath, opath, m, "1", "--cells-out", kpath], check=True, timeout=120, env=env)
+            cells[m] = np.fromfile(kpath, dtype="<f8")
+            out = cf.read_output(opath, case)
+            for step, f in ref["fields"].items():
+                assert np.array_equal(out["fields"][step], f), f"{name} {m} step {step}"
+    assert cells["lod"].size == case.nsteps * case.init.size
+    assert np.array_equal(cells[mode], cells["lod"])
