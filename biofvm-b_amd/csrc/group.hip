@@ -155,12 +155,15 @@ int sync_descriptors(Group *g)
         return rc;
     if (!g->d_prefix && (rc = hip_check(g, hipMalloc((void **)&g->d_prefix, flat.size() * sizeof(long long)), "prefixes")))
         return rc;
+    // the host copies live until the stream synchronisation below
+    std::vector<DomainDesc<double>> vd;
+    std::vector<DomainDesc<float>> vf;
     if (g->precision == BFVMB_FP64) {
-        auto v = build_descs<double>(g);
-        rc = hip_check(g, hipMemcpyAsync(g->d_desc, v.data(), desc_bytes, hipMemcpyHostToDevice, g->stream), "descriptors");
+        vd = build_descs<double>(g);
+        rc = hip_check(g, hipMemcpyAsync(g->d_desc, vd.data(), desc_bytes, hipMemcpyHostToDevice, g->stream), "descriptors");
     } else {
-        auto v = build_descs<float>(g);
-        rc = hip_check(g, hipMemcpyAsync(g->d_desc, v.data(), desc_bytes, hipMemcpyHostToDevice, g->stream), "descriptors");
+        vf = build_descs<float>(g);
+        rc = hip_check(g, hipMemcpyAsync(g->d_desc, vf.data(), desc_bytes, hipMemcpyHostToDevice, g->stream), "descriptors");
     }
     if (!rc)
         rc = hip_check(g, hipMemcpyAsync(g->d_prefix, flat.data(), flat.size() * sizeof(long long), hipMemcpyHostToDevice,
@@ -179,6 +182,8 @@ int sync_descriptors(Group *g)
     }
     if (!rc)
         rc = hip_check(g, hipStreamSynchronize(g->stream), "descriptor sync");
+    else
+        (void)hipStreamSynchronize(g->stream);  // copies queued before the error use the host vectors
     if (rc)
         return rc;
     int densest = 0;
