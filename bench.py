@@ -361,6 +361,8 @@ def main():
                     help="with --c5-only: the library's default launch shapes instead of the autotuner's")
     ap.add_argument("--batched-only", action="store_true",
                     help="time only the batched workload and print its JSON object (for profiling)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="time only the adjoint workload and print its JSON object (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device copy (reference for HBM byte counters)")
     ap.add_argument("--copy-ceiling", action="store_true",
@@ -388,6 +390,9 @@ def main():
         return
     if args.batched_only:
         print(json.dumps(bench_batched(torch, stream)), flush=True)
+        return
+    if args.adjoint_only:
+        print(json.dumps(bench_adjoint(torch, stream)), flush=True)
         return
     log("generating inputs ...")
     inputs = make_inputs()
