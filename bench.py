@@ -189,6 +189,9 @@ def bench_adjoint(torch, stream, n=128, nsteps=50, every=10, nagents=10_000, see
     target = rng.lognormal(0.0, 0.5, (n, n, n, ns))
     vs = n ** 3 * ns * nsteps
     out = {"workload": f"{n}^3x{ns}, {nagents} agents, {nsteps} steps, checkpoint every {every}"}
+    # launch shapes by the autotuner: a sweep here has 512 tiles, fewer than the default
+    # shapes' wavefront slots, and fewer wavefronts per CU keep whole 128-column lines in LDS
+    out["autotune"] = s.autotune().tolist()
     # fields resident in HBM (CUDA tensors: seed and gradients never cross PCIe), then the
     # same through host arrays
     init_d = torch.tensor(init, device="cuda:0")

@@ -1047,16 +1047,6 @@ Shape shape_of(const Group *g, int axis)
     // strided grid), and y/z keep 32 rows per lane in flight
     sh.bpc = u.lds_blocks_per_cu > 0 ? u.lds_blocks_per_cu : (axis == 0 && g->elem == 8 ? 6 : 4);
     sh.ring = u.ring > 0 ? u.ring : (axis == 0 ? 0 : 32);  // x: the variant's chunks in flight
-    // A default launch with fewer tiles than those wavefront slots would leave LDS unused:
-    // each block then gets the LDS of the slots the launch does not fill (fewer wavefronts
-    // per CU, at least two), so that more of each line stays on chip.  Measured on the
-    // adjoint workload at 128^3 x 2 (512 tiles per sweep): the FP64 x sweep 39 -> 29 us, its
-    // 128-column lines wholly in LDS.
-    if (u.lds_blocks_per_cu <= 0 && g->cu_count > 0 && !g->prefix[axis + 1].empty()) {
-        const long long nt = g->prefix[axis + 1].back();
-        if (nt > 0 && nt < (long long)g->cu_count * sh.bpc)
-            sh.bpc = std::max(2, (int)((nt + g->cu_count - 1) / g->cu_count));
-    }
     return sh;
 }
 
