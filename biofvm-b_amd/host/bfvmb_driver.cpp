@@ -11,7 +11,7 @@
 //
 // usage: bfvmb_driver <case.bin> <out.bin> [fp64|fp32] [--fused] [--time] [--mat out.mat]
 //                     [--bulk rates.bin] [--sample samples.bin] [--dnodes nodes.bin]
-//                     [--moves moves.bin moves_out.bin]
+//                     [--moves moves.bin moves_out.bin] [--autotune]
 //        bfvmb_driver --bench-dropin <n> <nagents> <steps> [fp64|fp32]
 // --bench-dropin: the drop-in workload (n^3 voxels x 4 substrates, agents in a centred sphere
 // of radius 3 mm * n/400, lognormal(0, 0.5) fields; drawn with std::mt19937_64, for timing)
@@ -148,7 +148,7 @@ int main(int argc, char **argv)
         return 2;
     }
     int precision = BFVMB_FP64;
-    bool fused = false, timing = false;
+    bool fused = false, timing = false, autotune = false;  // --autotune: M.autotune() before the steps
     const char *mat_path = nullptr;   // --mat PATH: write_to_matlab after the last step
     const char *bulk_path = nullptr;  // --bulk PATH: supply, target, uptake [nvox][nsub] doubles
     const char *sample_path = nullptr;  // --sample PATH: densities at the agents' voxels per snapshot
@@ -181,6 +181,8 @@ int main(int argc, char **argv)
             fused = true;
         else if (!std::strcmp(argv[i], "--time"))
             timing = true;
+        else if (!std::strcmp(argv[i], "--autotune"))
+            autotune = true;
     }
     FILE *f = std::fopen(argv[1], "rb");
     if (!f) {
@@ -353,6 +355,8 @@ int main(int argc, char **argv)
                 }
             }
         };
+        if (autotune)
+            M.autotune();  // at the first solver call
         int next = 0, done = 0;
         double ms = 0.0;
         for (int s = 1; s <= nsteps; ++s) {

@@ -73,6 +73,7 @@ struct PluginState {
     // x sweep); every other entry point runs a queued update first.  Not with bulk rates set,
     // since bfvmb_run would apply them again.
     bool agents_pending = false;
+    bool tune_pending = false;  // bfvmb_plugin_autotune: tune at the next solver call
 };
 
 std::map<Microenvironment *, PluginState> g_state;
@@ -235,6 +236,14 @@ void bfvmb_hip_LOD_3D(Microenvironment &M, double dt, int mpi_Size, int mpi_Rank
     S->rescan_dirichlet = false;
     if (!to_device(M, *S))
         return;
+    if (S->tune_pending) {  // times the shapes on the current field, then restores it
+        S->tune_pending = false;
+        int chosen[9];
+        if (bfvmb_autotune(S->ctx, 3, chosen) != BFVMB_OK) {
+            fail("autotune", S->ctx);
+            return;
+        }
+    }
     if (S->agents_pending) {  // the queued cell update and this step, as bfvmb_run launches them
         S->agents_pending = false;
         if (bfvmb_run(S->ctx, 1) != BFVMB_OK) {
@@ -419,6 +428,13 @@ void bfvmb_plugin_write_to_matlab(Microenvironment &M, const std::string &filena
         return;
     }
     M.write_to_matlab(filename);
+}
+
+void bfvmb_plugin_autotune(Microenvironment &M)
+{
+    PluginState *S = state_for(M);
+    if (S)
+        S->tune_pending = true;
 }
 
 void bfvmb_plugin_release(Microenvironment &M)

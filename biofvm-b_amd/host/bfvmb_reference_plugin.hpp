@@ -43,5 +43,10 @@ void bfvmb_plugin_finish(BioFVM::Microenvironment &M);      // wait until the GP
 void bfvmb_plugin_set_agents_resident(BioFVM::Microenvironment &M, bool resident);
 void bfvmb_plugin_invalidate_agents(BioFVM::Microenvironment &M);
 void bfvmb_plugin_release(BioFVM::Microenvironment &M);     // free the device context
+// The next bfvmb_hip_LOD_3D call first times the sweeps' launch shapes on this
+// microenvironment's field (bfvmb_autotune) and keeps the fastest for the calls after it; the
+// field is restored after the timing and results do not depend on the shapes.  The library's
+// defaults are tuned at 400^3 x 4; other grids can run faster at other shapes.
+void bfvmb_plugin_autotune(BioFVM::Microenvironment &M);
 // Microenvironment::write_to_matlab, from the device when it holds the newer field
 void bfvmb_plugin_write_to_matlab(BioFVM::Microenvironment &M, const std::string &filename);

@@ -387,6 +387,11 @@ void Microenvironment::ensure_solver(double dt)
     }
     push_dirichlet();
     sync_to_device();
+    if (tune_pending_) {  // on the current field, which the autotuner restores
+        tune_pending_ = false;
+        int chosen[9];
+        check(bfvmb_autotune(ctx_, 3, chosen), "bfvmb_autotune");
+    }
 }
 
 void Microenvironment::simulate_diffusion_decay(double dt)

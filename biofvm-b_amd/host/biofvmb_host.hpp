@@ -162,6 +162,10 @@ public:
     void simulate_cell_sources_and_sinks(double dt) { simulate_cell_sources_and_sinks(all_basic_agents, dt); }
     // nsteps of: sources/sinks then LOD (complete_diffusion_demo.cpp:489-493), on device
     void run(std::vector<Basic_Agent *> &agents, double dt, int nsteps);
+    // the next solver call (simulate_diffusion_decay or run) first times the sweeps' launch
+    // shapes on the current field (bfvmb_autotune, which restores the field) and keeps the
+    // fastest; results do not depend on the shapes.  The defaults are tuned at 400^3 x 4.
+    void autotune() { tune_pending_ = true; }
 
     // the context, for direct C-ABI calls (a waiting cell update runs first)
     bfvmb_ctx *context()
@@ -200,6 +204,7 @@ private:
     // x sweep's launch); every other method that reaches the device runs a waiting update
     // first.  Not once bulk rates are on the device (bfvmb_run would apply them again).
     bool agents_pending_ = false;
+    bool tune_pending_ = false;  // autotune() at the next solver call
     bfvmb_ctx *ctx_ = nullptr;
 };
 

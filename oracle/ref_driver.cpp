@@ -14,7 +14,7 @@
 //
 // usage: ref_driver <case.bin> <out.bin> [lod|avx] [granularity] [--time|--time-pipelined] [--mat out.mat]
 //                   [--bulk rates.bin] [--grad grad.bin] [--dnodes nodes.bin] [--serial-agents]
-//                   [--moves moves.bin moves_out.bin]
+//                   [--moves moves.bin moves_out.bin] [--autotune]
 //        ref_driver_hip <case.bin> <out.bin> [hip|hipres|hipall] ...   (BFVMB_PLUGIN build: the
 //        GPU solver in the diffusion_decay_solver slot; hipres keeps the field resident, hipall
 //        the field and the agents)
@@ -99,9 +99,13 @@ int main(int argc, char **argv)
                             // simulate_bulk_sources_and_sinks runs before the cell update
     bool serial_agents = false;  // --serial-agents: simulate_cell_sources_and_sinks on one thread (its
                                  // omp loop races where agents share a voxel), the LOD on all
-    for (int i = 3; i < argc; ++i)
+    bool autotune = false;  // --autotune (GPU slot): bfvmb_plugin_autotune before the first step
+    for (int i = 3; i < argc; ++i) {
         if (std::string(argv[i]) == "--serial-agents")
             serial_agents = true;
+        if (std::string(argv[i]) == "--autotune")
+            autotune = true;
+    }
     // --moves IN OUT: agent moves, IN = nmoves, then per move: step, route, positions [n][3];
     // before the cell update of that step every agent moves by assign_position (route 0) or by
     // setting its position and calling update_voxel_index (route 1); OUT gets, per move, the
@@ -357,6 +361,8 @@ int main(int argc, char **argv)
             bfvmb_plugin_set_resident(M, true);
         if (solver == "hipall" && s == 1)
             bfvmb_plugin_set_agents_resident(M, true);
+        if (hip && autotune && s == 1)
+            bfvmb_plugin_autotune(M);  // at this step's solver call
 #endif
         M.simulate_diffusion_decay(dt, size, rank, coords, dims, cart);
 #ifdef BFVMB_PLUGIN

@@ -39,6 +39,23 @@ def test_host_driver_matches_reference(name, mode):
 
 
 @pytest.mark.parametrize("mode", ["per_call", "fused"])
+@pytest.mark.parametrize("name", ["aniso_17x23x31x3", "multi_24x20x28x5", "long_x_300x17x19x1"])
+def test_host_driver_autotuned(name, mode):
+    """Microenvironment::autotune() before the first step (timed at the first solver call, on
+    the live field, with a cell update waiting in per-call mode): bit for bit the reference."""
+    case, ref = golden.load(name)
+    print(f"[gpu] host driver autotuned {name} {mode}", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        cpath, opath = os.path.join(d, "case.bin"), os.path.join(d, "out.bin")
+        case.write(cpath)
+        args = [DRIVER, cpath, opath, "fp64", "--autotune"] + (["--fused"] if mode == "fused" else [])
+        subprocess.run(args, check=True, timeout=120)
+        out = cf.read_output(opath, case)
+    for step, f in ref["fields"].items():
+        assert np.array_equal(out["fields"][step], f), f"{name} step {step}: {(out['fields'][step] != f).sum()} differ"
+
+
+@pytest.mark.parametrize("mode", ["per_call", "fused"])
 @pytest.mark.parametrize("name", golden.MAT_NAMES)
 def test_host_driver_writes_reference_mat_file(name, mode):
     """Microenvironment::write_to_matlab of the host layer (written from the device) equals

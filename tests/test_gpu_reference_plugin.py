@@ -41,6 +41,29 @@ def test_reference_driver_with_gpu_slot(name, mode):
 
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="reference build with the plugin (make -C oracle ref-hip) absent")
 @pytest.mark.parametrize("mode", ["hip", "hipres", "hipall"])
+@pytest.mark.parametrize("name", ["aniso_17x23x31x3", "multi_24x20x28x5", "long_x_300x17x19x1", "dnodes_17x23x31x3"])
+def test_reference_driver_autotuned(name, mode):
+    """bfvmb_plugin_autotune before the first step: that step's solver call times the launch
+    shapes on the live field (with a cell update waiting, in the resident modes) and restores
+    it; every step then runs at the chosen shapes and stays bit for bit the reference's."""
+    case, ref = golden.load(name)
+    print(f"[gpu] reference driver + plugin, autotuned, {name} {mode}", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        cpath, opath, npath = (os.path.join(d, f) for f in ("case.bin", "out.bin", "nodes.bin"))
+        case.write(cpath)
+        extra = ["--autotune"]
+        if case.dnodes is not None:
+            case.write_dnodes(npath)
+            extra += ["--dnodes", npath]
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([DRIVER, cpath, opath, mode, "1"] + extra, check=True, timeout=120, env=env)
+        out = cf.read_output(opath, case)
+    for step, f in ref["fields"].items():
+        assert np.array_equal(out["fields"][step], f), f"{name} step {step}: {(out['fields'][step] != f).sum()} differ"
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="reference build with the plugin (make -C oracle ref-hip) absent")
+@pytest.mark.parametrize("mode", ["hip", "hipres", "hipall"])
 @pytest.mark.parametrize("name", golden.MAT_NAMES)
 def test_reference_driver_mat_file(name, mode):
     """bfvmb_plugin_write_to_matlab in the reference's own driver (from the device when the
