@@ -50,6 +50,7 @@ template struct Expose<VolumeChanged, &Basic_Agent::volume_is_changed>;
 struct PluginState {
     bfvmb_ctx *ctx = nullptr;
     int nx = 0, ny = 0, nz = 0, nsub = 0;
+    int precision = BFVMB_FP64;  // bfvmb_plugin_set_precision
     bool coef_done = false;
     bool resident = false;
     bool device_current = false;  // the device holds the newest field
@@ -154,10 +155,12 @@ PluginState *state_for(Microenvironment &M)
     const int nz = (int)M.mesh.z_coordinates.size(), ns = M.number_of_densities();
     if (S.ctx && (S.nx != nx || S.ny != ny || S.nz != nz || S.nsub != ns)) {
         bfvmb_destroy(S.ctx);
+        const int precision = S.precision;
         S = PluginState();
+        S.precision = precision;
     }
     if (!S.ctx) {
-        if (bfvmb_create(&S.ctx, nx, ny, nz, ns, M.mesh.dx, BFVMB_FP64, 0) != BFVMB_OK) {
+        if (bfvmb_create(&S.ctx, nx, ny, nz, ns, M.mesh.dx, S.precision, 0) != BFVMB_OK) {
             fail("bfvmb_create", nullptr);
             S.ctx = nullptr;
             return nullptr;
@@ -428,6 +431,21 @@ void bfvmb_plugin_write_to_matlab(Microenvironment &M, const std::string &filena
         return;
     }
     M.write_to_matlab(filename);
+}
+
+void bfvmb_plugin_set_precision(Microenvironment &M, int precision)
+{
+    if (precision != BFVMB_FP64 && precision != BFVMB_FP32) {
+        std::printf("Error: bfvmb plugin: set_precision: BFVMB_FP64 or BFVMB_FP32\n");
+        return;
+    }
+    PluginState &S = g_state[&M];
+    if (S.ctx && S.precision != precision) {  // the field goes back to the host, then a new context
+        bfvmb_plugin_sync(M);
+        bfvmb_destroy(S.ctx);
+        S = PluginState();
+    }
+    S.precision = precision;
 }
 
 void bfvmb_plugin_autotune(Microenvironment &M)

@@ -24,6 +24,8 @@
 #include <string>
 #include <vector>
 
+#include "biofvmb.h"  // BFVMB_FP64 / BFVMB_FP32
+
 void bfvmb_hip_LOD_3D(BioFVM::Microenvironment &M, double dt, int mpi_Size, int mpi_Rank, int *mpi_Coords,
                       int *mpi_Dims, MPI_Comm mpi_Cart_comm);
 void bfvmb_hip_cell_sources_and_sinks(BioFVM::Microenvironment &M, std::vector<BioFVM::Basic_Agent *> &agents,
@@ -48,5 +50,10 @@ void bfvmb_plugin_release(BioFVM::Microenvironment &M);     // free the device c
 // field is restored after the timing and results do not depend on the shapes.  The library's
 // defaults are tuned at 400^3 x 4; other grids can run faster at other shapes.
 void bfvmb_plugin_autotune(BioFVM::Microenvironment &M);
+// BFVMB_FP32: the field is stored and swept in FP32 (FP64 recurrence, one rounding per store;
+// within 1e-5 of the reference instead of bit-identical, about 1.9x the FP64 throughput).
+// Call before the other plugin calls for M: a context of the other precision is released
+// (its field synced to p_density_vectors first) and the settings made for it are dropped.
+void bfvmb_plugin_set_precision(BioFVM::Microenvironment &M, int precision);
 // Microenvironment::write_to_matlab, from the device when it holds the newer field
 void bfvmb_plugin_write_to_matlab(BioFVM::Microenvironment &M, const std::string &filename);

@@ -14,7 +14,7 @@
 //
 // usage: ref_driver <case.bin> <out.bin> [lod|avx] [granularity] [--time|--time-pipelined] [--mat out.mat]
 //                   [--bulk rates.bin] [--grad grad.bin] [--dnodes nodes.bin] [--serial-agents]
-//                   [--moves moves.bin moves_out.bin] [--autotune]
+//                   [--moves moves.bin moves_out.bin] [--autotune] [--fp32]
 //        ref_driver_hip <case.bin> <out.bin> [hip|hipres|hipall] ...   (BFVMB_PLUGIN build: the
 //        GPU solver in the diffusion_decay_solver slot; hipres keeps the field resident, hipall
 //        the field and the agents)
@@ -100,11 +100,14 @@ int main(int argc, char **argv)
     bool serial_agents = false;  // --serial-agents: simulate_cell_sources_and_sinks on one thread (its
                                  // omp loop races where agents share a voxel), the LOD on all
     bool autotune = false;  // --autotune (GPU slot): bfvmb_plugin_autotune before the first step
+    bool fp32 = false;      // --fp32 (GPU slot): bfvmb_plugin_set_precision(M, BFVMB_FP32) first
     for (int i = 3; i < argc; ++i) {
         if (std::string(argv[i]) == "--serial-agents")
             serial_agents = true;
         if (std::string(argv[i]) == "--autotune")
             autotune = true;
+        if (std::string(argv[i]) == "--fp32")
+            fp32 = true;
     }
     // --moves IN OUT: agent moves, IN = nmoves, then per move: step, route, positions [n][3];
     // before the cell update of that step every agent moves by assign_position (route 0) or by
@@ -218,6 +221,8 @@ int main(int argc, char **argv)
 #ifdef BFVMB_PLUGIN
     if (hip)
         M.diffusion_decay_solver = bfvmb_hip_LOD_3D;
+    if (hip && fp32)
+        bfvmb_plugin_set_precision(M, BFVMB_FP32);
 #else
     if (hip) {
         fprintf(stderr, "built without the GPU plugin\n");

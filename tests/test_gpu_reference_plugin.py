@@ -64,6 +64,32 @@ def test_reference_driver_autotuned(name, mode):
 
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="reference build with the plugin (make -C oracle ref-hip) absent")
 @pytest.mark.parametrize("mode", ["hip", "hipres", "hipall"])
+@pytest.mark.parametrize("name", ["aniso_17x23x31x3", "multi_24x20x28x5", "dnodes_17x23x31x3", "plain_50cube"])
+def test_reference_driver_fp32_storage(name, mode):
+    """bfvmb_plugin_set_precision(M, BFVMB_FP32) in the reference's own driver: the field is
+    stored in FP32 on the device and every snapshot stays within 1e-5 (max relative error) of
+    the reference's FP64 solver."""
+    case, ref = golden.load(name)
+    print(f"[gpu] reference driver + plugin, FP32 storage, {name} {mode}", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        cpath, opath, npath = (os.path.join(d, f) for f in ("case.bin", "out.bin", "nodes.bin"))
+        case.write(cpath)
+        extra = ["--fp32"]
+        if case.dnodes is not None:
+            case.write_dnodes(npath)
+            extra += ["--dnodes", npath]
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([DRIVER, cpath, opath, mode, "1"] + extra, check=True, timeout=120, env=env)
+        out = cf.read_output(opath, case)
+    for step, f in ref["fields"].items():
+        r = np.abs(out["fields"][step] - f) / np.maximum(np.abs(f), 1e-300)
+        print(f"{name} {mode} step {step}: FP32 max rel {r.max():.3e}, RMS {np.sqrt(np.mean(r * r)):.3e}")
+        assert r.max() <= 1e-5, f"{name} step {step}"
+    assert not np.array_equal(out["fields"][max(ref["fields"])], ref["fields"][max(ref["fields"])])  # FP32 ran
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="reference build with the plugin (make -C oracle ref-hip) absent")
+@pytest.mark.parametrize("mode", ["hip", "hipres", "hipall"])
 @pytest.mark.parametrize("name", golden.MAT_NAMES)
 def test_reference_driver_mat_file(name, mode):
     """bfvmb_plugin_write_to_matlab in the reference's own driver (from the device when the
