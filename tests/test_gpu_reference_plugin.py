@@ -90,6 +90,27 @@ def test_reference_driver_fp32_storage(name, mode):
 
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="reference build with the plugin (make -C oracle ref-hip) absent")
 @pytest.mark.parametrize("mode", ["hip", "hipres", "hipall"])
+@pytest.mark.parametrize("name", golden.BULK_NAMES)
+def test_reference_driver_fp32_storage_bulk(name, mode):
+    """FP32 storage with the bulk slot (bfvmb_hip_bulk_sources_and_sinks) in the reference's
+    driver: within 1e-5 of the reference's FP64 run."""
+    case, ref = golden.load(name)
+    print(f"[gpu] reference driver + plugin, FP32 storage, bulk {name} {mode}", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        cpath, opath, bpath = (os.path.join(d, f) for f in ("case.bin", "out.bin", "bulk.bin"))
+        case.write(cpath)
+        case.write_bulk(bpath)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([DRIVER, cpath, opath, mode, "1", "--bulk", bpath, "--fp32"], check=True, timeout=120, env=env)
+        out = cf.read_output(opath, case)
+    for step, f in ref["fields"].items():
+        r = np.abs(out["fields"][step] - f) / np.maximum(np.abs(f), 1e-300)
+        print(f"{name} {mode} step {step}: FP32 max rel {r.max():.3e}")
+        assert r.max() <= 1e-5, f"{name} step {step}"
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="reference build with the plugin (make -C oracle ref-hip) absent")
+@pytest.mark.parametrize("mode", ["hip", "hipres", "hipall"])
 @pytest.mark.parametrize("name", golden.MAT_NAMES)
 def test_reference_driver_mat_file(name, mode):
     """bfvmb_plugin_write_to_matlab in the reference's own driver (from the device when the
