@@ -63,8 +63,12 @@ def traffic(fetch_db, write_db, out):
         big = max(vals) if vals else None
         cal[label] = COPY_BYTES / big if big else 1.0
     res, rows = {}, []
-    # k_sweep_x<T, PC, REGC, TL, VF, AG>: AG = the step's agent update in the same launch
+    # k_sweep_x<T, PC, REGC, TL, VF, AG[, NT]>: AG = the step's agent update in the same launch
     phase = {"k_sweep_x": "sweep_x", "k_sweep_yz<double, 1": "sweep_y", "k_sweep_yz<double, 2": "sweep_z"}
+
+    def template_args(name):
+        return name[name.index("<") + 1:name.rindex(">")].split(", ") if "<" in name else []
+
     for k in f:
         s = short(k)
         if "k_sweep" not in s and "k_agents" not in s:
@@ -74,7 +78,8 @@ def traffic(fetch_db, write_db, out):
         rows.append((s, fb, wb))
         for pre, ph in phase.items():
             if s.startswith(pre) and "double" in s:
-                if ph == "sweep_x" and s.endswith(", true>"):
+                args = template_args(s)
+                if ph == "sweep_x" and len(args) >= 6 and args[5] == "true":
                     ph = "sweep_x_with_agents"
                 res[ph] = fb + wb
     # provenance: the sweep source these counters describe (bench.py reports the traffic only
