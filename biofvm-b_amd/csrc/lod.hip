@@ -57,10 +57,11 @@ __device__ __forceinline__ gmem<T> as_global(T *p)
 }
 
 // A sweep's results, which nothing in the sweep reads again, are stored non-temporally where
-// each store instruction writes whole 128-byte lines: the L2 then evicts those lines first
-// and keeps what is read again (the recomputed rows, the agent pre-pass's lines).  Measured
-// at 400^3 x 4 (us per sweep, results bit-identical): FP64 x 1211 -> 1144, y 911 -> 858,
-// z 962 -> 916; FP32 y 451 -> 432, z 474 -> 448.  Where a store covers parts of lines (rows
+// each store instruction writes whole 128-byte lines.  The HBM bytes stay the same (FETCH_SIZE
+// and WRITE_SIZE per launch unchanged at 400^3 x 4), but the sweeps move them faster (FP64 x
+// with the agent update 5.08 -> 5.45 TB/s).  Measured at 400^3 x 4 (us per sweep, results
+// bit-identical): FP64 x 1211 -> 1144, y 911 -> 858, z 962 -> 916; FP32 y 451 -> 432, z 474
+// -> 448.  Where a store covers parts of lines (rows
 // that do not start on a line: most members of a batch of uneven domains, FP64 11.6 ->
 // 13.2 ms per batch step with non-temporal stores everywhere; the FP32 x sweep's 64-byte
 // halves of lines: 648 -> 696 us), partial lines leave the L2 before their other part is
