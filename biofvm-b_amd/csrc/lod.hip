@@ -1075,15 +1075,18 @@ Shape shape_of(const Group *g, int axis)
 
 // whether every domain's rows (x: FP64 only, whose 16-column row segments are whole lines;
 // y: rows of nx elements) or planes (z: nx * ny elements) are whole 128-byte lines, so that
-// each result store of the sweep writes whole lines (non-temporal stores, store_result)
+// each result store of the sweep writes whole lines (non-temporal stores, store_result).  The
+// FP32 y/z sweeps (256-byte row segments) also gain where rows are whole 64-byte halves of
+// lines: FP32 y at 400^3 x 4 (1600-byte rows) 451 -> 432 us.
 template <class T>
 bool nt_rows(const Group *g, int axis)
 {
     if (axis == 0 && sizeof(T) != 8)
         return false;
+    const size_t unit = axis != 0 && sizeof(T) == 4 ? 64 : 128;
     for (const Domain &dm : g->doms) {
         const size_t row = axis == 2 ? (size_t)dm.nx * dm.ny : (size_t)dm.nx;
-        if (row * sizeof(T) % 128 != 0)
+        if (row * sizeof(T) % unit != 0)
             return false;
     }
     return true;
